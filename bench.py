@@ -1,0 +1,142 @@
+"""Benchmark: RBF randomised LRA on one MI355X.
+
+One step = one LRA solve of the n = 65536, d = 128, k = 256, p = 16, q = 2
+Gaussian-mixture workload (16 centres in [-1,1]^128, sigma = 0.25, gamma = 0.5/d):
+Y = K*Omega, CholeskyQR2, two power iterations, W = K*Q (B = W^T), inputs
+resident in HBM.  GFLOP/s counts F_MFMA + F_orth:
+    F_MFMA = (q+2) * (2 n^2 d + 2 n^2 l),  F_orth = (q+1) * (6 n l^2 + 2/3 l^3)
+Prints ONE JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_FP64_MFMA = 78.6e12  # MI355X dense FP64 matrix, FLOP/s
+PEAK_BF16_MFMA = 2.5e15  # MI355X dense BF16 matrix, FLOP/s
+
+
+def lra_flops(n, d, l, q):
+    f_mfma = (q + 2) * (2.0 * n * n * d + 2.0 * n * n * l)
+    f_orth = (q + 1) * (6.0 * n * l * l + 2.0 / 3.0 * l**3)
+    return f_mfma, f_orth
+
+
+def cpu_baseline(X, Om, gamma, rows, threads):
+    """Oracle (FP64 CPU restatement) on a bounded sample of the same workload:
+    rows [0, rows) of one K*Omega pass, i.e. 2*rows*n*(d+l) flops."""
+    import oracle as orc
+    orc.set_threads(threads)
+    n, d = X.shape
+    l = Om.shape[1]
+    Xi = np.ascontiguousarray(X[:rows])
+    t0 = time.perf_counter()
+    orc.rbf_project(Xi, X, Om, gamma)
+    dt = time.perf_counter() - t0
+    flops = 2.0 * rows * n * (d + l)
+    return {"value": flops / dt / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": "port",
+            "sample": f"rows 0..{rows} of one K*Omega pass (m={rows}, n={n}, d={d}, l={l}), "
+                      f"FP64 oracle, {dt:.2f} s"}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=65536)
+    ap.add_argument("--d", type=int, default=128)
+    ap.add_argument("--k", type=int, default=256)
+    ap.add_argument("--p", type=int, default=16)
+    ap.add_argument("--q", type=int, default=2)
+    ap.add_argument("--cpu-rows", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-error", action="store_true")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
+              "the multi-GPU sharding is a design note, not a code path", file=sys.stderr)
+        return 2
+
+    import torch
+    import darray_optimizer_amd as dh
+
+    n, d, l, q = a.n, a.d, a.k + a.p, a.q
+    gamma = 0.5 / d
+    X = dh.fill_mixture(0, d, n, 16, 0.25)
+    Om = np.asfortranarray(dh.fill_normal(1, n, l))
+    h = dh.Handle(0)
+    stream = torch.cuda.current_stream()
+    Xd = torch.from_numpy(X).cuda()
+    Omd = dh.to_dev(Om)
+    Q = torch.empty((l, n), dtype=torch.float64, device="cuda")
+    W = torch.empty((l, n), dtype=torch.float64, device="cuda")
+    f_mfma, f_orth = lra_flops(n, d, l, q)
+
+    def solve(prec):
+        st = h.lra(Xd, Omd, gamma, q, Q, W, prec=prec)
+        assert st == 0, st
+
+    def timed(prec, steps, warmup):
+        for _ in range(warmup):
+            solve(prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            solve(prec)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    s64 = timed(dh.DH_PREC_F64, a.steps, a.warmup)
+    res64 = {"s_per_solve": s64, "gflops": (f_mfma + f_orth) / s64 / 1e9}
+    if not a.no_error:
+        e2, k2 = h.lra_error(Xd, gamma, Q, W)
+        res64["rel_err"] = float(np.sqrt(max(e2, 0.0) / k2))
+
+    # Dominant kernel: one fused FP64 K*Omega pass, timed with events on the
+    # stream it runs on.
+    Yd = torch.empty((l, n), dtype=torch.float64, device="cuda")
+    h.rbf_project_f64(Xd, Xd, Omd, gamma, Yd)
+    reps = 3
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        h.rbf_project_f64(Xd, Xd, Omd, gamma, Yd)
+    e1.record(stream)
+    e1.synchronize()
+    t_pass = e0.elapsed_time(e1) / 1e3 / reps
+    pass_flops = 2.0 * n * n * d + 2.0 * n * n * l
+    achieved = pass_flops / t_pass
+    roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64_MFMA / 1e12,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA, "traffic": None,
+                "kernel": "rbf_project_f64_kernel<32,17>", "kernel_ms": t_pass * 1e3}
+
+    line = {
+        "metric": "RBF-LRA GFLOP/s & s/solve, mixed vs FP64 on 1 MI355X at matched ||K-QB||/||K||",
+        "value": res64["gflops"], "unit": "GFLOP/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": s64 * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "rbf_lra_gaussian_mixture", "n": n, "d": d, "k": a.k, "p": a.p,
+                   "q": q, "l": l, "gamma": gamma, "centres": 16, "sigma": 0.25,
+                   "precision": "fp64"},
+        "fp64": res64, "roofline": roofline,
+    }
+    if not a.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        line["cpu_baseline"] = cpu_baseline(X, Om, gamma, a.cpu_rows, threads)
+    print(json.dumps(line), flush=True)
+    h.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

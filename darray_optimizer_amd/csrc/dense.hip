@@ -1,0 +1,541 @@
+// Dense building blocks on gfx950 MFMA: GEMM / SYRK (f32 and f64), small
+// in-LDS Cholesky and triangular-solve kernels, and the blocked drivers of
+// the Cholesky and TRSM that DArray runs through BLAS/LAPACK:
+//   multiply_matrix / matrix_multiply  (cblas_like.h:39-60, blas_like.h:62-93)
+//   symmetric_update -> ssyrk_         (cblas_like.h:61-69)
+//   triangular_solve                   (blas_like.h:95-128)
+//   cholesky_factorize                 (lapack_like.h:9-56)
+//
+// One template serves both precisions: the f64 and f32 16x16x4 MFMAs share
+// an operand layout (mfma_tile.h).
+#include <cstdio>
+
+#include "dh_common.h"
+#include "mfma_tile.h"
+
+// ------------------------------------------------------------------ GEMM
+//
+// C tile BM x BN per 256-thread workgroup (tile_mma in mfma_tile.h).
+// blockIdx.z selects a k-range of `kchunk`; with `part` != nullptr each
+// split writes its raw product to part[z] (m x n, ld m) and a second kernel
+// reduces.  uplo_upper skips tiles strictly below the diagonal and never
+// writes an element with i > j (SYRK semantics).  `stop` (nullable) is a
+// device status word: nonzero makes the kernel return at once.
+template <typename T, int BM, int BN, bool TA, bool TB>
+__global__ __launch_bounds__(256) void gemm_kernel(int m, int n, int k, T alpha,
+                                                    const T* __restrict__ A, int lda,
+                                                    const T* __restrict__ B, int ldb, T beta,
+                                                    T* __restrict__ C, int ldc, int kchunk,
+                                                    int uplo_upper, T* __restrict__ part,
+                                                    const int* __restrict__ stop) {
+    constexpr int FM = BM / 32, FN = BN / 32;
+    __shared__ T sA[kTileBK][BM + kTilePad];
+    __shared__ T sB[kTileBK][BN + kTilePad];
+    if (stop && *stop) return;
+    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    if (uplo_upper && m0 >= n0 + BN) return;
+    const int kbeg = blockIdx.z * kchunk;
+    const int kend = min(k, kbeg + kchunk);
+    typename Mfma<T>::acc_t acc[FM][FN];
+    zero_acc<T, FM, FN>(acc);
+    tile_mma<T, BM, BN, TA, TB>(acc, sA, sB, m, n, m0, n0, kbeg, kend, A, lda, B, ldb);
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int i = m0 + tile_row<BM, BN>(a, r);
+                int j = n0 + tile_col<BM, BN>(b);
+                if (i < m && j < n && (!uplo_upper || i <= j)) {
+                    T v = acc[a][b][r];
+                    if (part) {
+                        part[(size_t)blockIdx.z * m * n + i + (size_t)j * m] = v;
+                    } else {
+                        T* c = C + i + (size_t)j * ldc;
+                        *c = (beta == T(0)) ? alpha * v : alpha * v + beta * (*c);
+                    }
+                }
+            }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(int m, int n, int nsplit,
+                                                             const T* __restrict__ part, T alpha,
+                                                             T beta, T* __restrict__ C, int ldc,
+                                                             int uplo_upper,
+                                                             const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    size_t mn = (size_t)m * n;
+    if (idx >= mn) return;
+    int i = (int)(idx % m), j = (int)(idx / m);
+    if (uplo_upper && i > j) return;
+    T s = 0;
+    for (int z = 0; z < nsplit; ++z) s += part[(size_t)z * mn + idx];
+    T* c = C + i + (size_t)j * ldc;
+    *c = (beta == T(0)) ? alpha * s : alpha * s + beta * (*c);
+}
+
+template <typename T, int BM, int BN>
+static int launch_gemm(dh_handle_t h, bool ta, bool tb, int m, int n, int k, T alpha, const T* A,
+                       int lda, const T* B, int ldb, T beta, T* C, int ldc, int uplo,
+                       const int* stop) {
+    int tm = dh_cdiv(m, BM), tn = dh_cdiv(n, BN);
+    long tiles = (long)tm * tn;
+    int nsplit = 1;
+    // Tall-skinny products (CholeskyQR Gram, TRSM updates with short outputs)
+    // have few output tiles and a long k: split k to fill the chip.
+    if (tiles < 2L * h->num_cu && k >= 1024) {
+        nsplit = (int)((2L * h->num_cu + tiles - 1) / tiles);
+        nsplit = min(nsplit, k / 256);
+        nsplit = max(nsplit, 1);
+        nsplit = min(nsplit, 64);
+    }
+    int kchunk = dh_cdiv(dh_cdiv(k, nsplit), 16) * 16;
+    if (kchunk <= 0) kchunk = 16;
+    nsplit = dh_cdiv(k, kchunk);
+    if (nsplit < 1) nsplit = 1;
+    T* part = nullptr;
+    if (nsplit > 1) {
+        part = (T*)dh_ws(h, DH_WS_SPLITK, sizeof(T) * (size_t)nsplit * m * n);
+        if (!part) return DH_ERR_ALLOC;
+    }
+    dim3 grid(tm, tn, nsplit);
+    if (!ta && !tb)
+        gemm_kernel<T, BM, BN, false, false><<<grid, 256, 0, h->stream>>>(
+            m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, kchunk, uplo, part, stop);
+    else if (!ta && tb)
+        gemm_kernel<T, BM, BN, false, true><<<grid, 256, 0, h->stream>>>(
+            m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, kchunk, uplo, part, stop);
+    else if (ta && !tb)
+        gemm_kernel<T, BM, BN, true, false><<<grid, 256, 0, h->stream>>>(
+            m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, kchunk, uplo, part, stop);
+    else
+        gemm_kernel<T, BM, BN, true, true><<<grid, 256, 0, h->stream>>>(
+            m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, kchunk, uplo, part, stop);
+    DH_CHECK_LAUNCH();
+    if (nsplit > 1) {
+        size_t mn = (size_t)m * n;
+        splitk_reduce_kernel<T><<<dh_cdiv((long)mn, 256), 256, 0, h->stream>>>(
+            m, n, nsplit, part, alpha, beta, C, ldc, uplo, stop);
+        DH_CHECK_LAUNCH();
+    }
+    return 0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void scale_kernel(int m, int n, T alpha, T* __restrict__ C,
+                                                     int ldc, int uplo_upper) {
+    size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (size_t)m * n) return;
+    int i = (int)(idx % m), j = (int)(idx / m);
+    if (uplo_upper && i > j) return;
+    T* c = C + i + (size_t)j * ldc;
+    *c = (alpha == T(0)) ? T(0) : alpha * (*c);
+}
+
+// Internal GEMM with an optional device stop word.
+template <typename T>
+static int gemm_stop(dh_handle_t h, bool ta, bool tb, int m, int n, int k, T alpha, const T* A,
+                     int lda, const T* B, int ldb, T beta, T* C, int ldc, int uplo,
+                     const int* stop) {
+    if (m == 0 || n == 0) return 0;
+    if (k == 0 || alpha == T(0)) {
+        if (beta == T(1)) return 0;
+        scale_kernel<T><<<dh_cdiv((long)m * n, 256), 256, 0, h->stream>>>(m, n, beta, C, ldc,
+                                                                           uplo);
+        DH_CHECK_LAUNCH();
+        return 0;
+    }
+    if ((long)m * n >= 2048L * 2048L)
+        return launch_gemm<T, 128, 128>(h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+                                        uplo, stop);
+    return launch_gemm<T, 64, 64>(h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, uplo,
+                                  stop);
+}
+
+template <typename T>
+int dh_gemm_impl(dh_handle_t h, bool ta, bool tb, int m, int n, int k, T alpha, const T* A,
+                 int lda, const T* B, int ldb, T beta, T* C, int ldc, int uplo_upper) {
+    return gemm_stop<T>(h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, uplo_upper,
+                        nullptr);
+}
+template int dh_gemm_impl<double>(dh_handle_t, bool, bool, int, int, int, double, const double*,
+                                  int, const double*, int, double, double*, int, int);
+template int dh_gemm_impl<float>(dh_handle_t, bool, bool, int, int, int, float, const float*,
+                                 int, const float*, int, float, float*, int, int);
+
+// ------------------------------------------------------ small triangles
+//
+// A triangle of order nb <= 64 lives in LDS; one thread owns one right-hand
+// side (a column for side L, a row for side R) in registers.
+
+constexpr int kSmall = 64;
+
+template <typename T>
+__device__ __forceinline__ void load_upper_lds(T (*s)[kSmall + 1], int nb, const T* A, int lda) {
+    for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
+        int i = e % nb, j = e / nb;
+        s[i][j] = (i <= j) ? A[i + (size_t)j * lda] : T(0);
+    }
+}
+
+// B := U^{-T} B  (forward substitution), B nb x ncols.
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_lut_small_kernel(int nb, int ncols,
+                                                              const T* __restrict__ A, int lda,
+                                                              T* __restrict__ B, int ldb,
+                                                              int unit,
+                                                              const int* __restrict__ stop) {
+    __shared__ T s[kSmall][kSmall + 1];
+    if (stop && *stop) return;
+    load_upper_lds<T>(s, nb, A, lda);
+    __syncthreads();
+    int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= ncols) return;
+    T* b = B + (size_t)col * ldb;
+    T x[kSmall];
+#pragma unroll
+    for (int i = 0; i < kSmall; ++i) x[i] = (i < nb) ? b[i] : T(0);
+#pragma unroll
+    for (int i = 0; i < kSmall; ++i) {
+        if (i < nb) {
+            T acc = x[i];
+#pragma unroll
+            for (int k = 0; k < i; ++k) acc -= s[k][i] * x[k];
+            x[i] = unit ? acc : acc / s[i][i];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kSmall; ++i)
+        if (i < nb) b[i] = x[i];
+}
+
+// B := U^{-1} B  (backward substitution), B nb x ncols.
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_lun_small_kernel(int nb, int ncols,
+                                                              const T* __restrict__ A, int lda,
+                                                              T* __restrict__ B, int ldb,
+                                                              int unit,
+                                                              const int* __restrict__ stop) {
+    __shared__ T s[kSmall][kSmall + 1];
+    if (stop && *stop) return;
+    load_upper_lds<T>(s, nb, A, lda);
+    __syncthreads();
+    int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= ncols) return;
+    T* b = B + (size_t)col * ldb;
+    T x[kSmall];
+#pragma unroll
+    for (int i = 0; i < kSmall; ++i) x[i] = (i < nb) ? b[i] : T(0);
+#pragma unroll
+    for (int i = kSmall - 1; i >= 0; --i) {
+        if (i < nb) {
+            T acc = x[i];
+#pragma unroll
+            for (int k = i + 1; k < kSmall; ++k)
+                if (k < nb) acc -= s[i][k] * x[k];
+            x[i] = unit ? acc : acc / s[i][i];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kSmall; ++i)
+        if (i < nb) b[i] = x[i];
+}
+
+// B := B U^{-1}, B nrows x nb (one row per thread; coalesced across threads).
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_run_small_kernel(int nb, int nrows,
+                                                              const T* __restrict__ A, int lda,
+                                                              T* __restrict__ B, int ldb,
+                                                              int unit,
+                                                              const int* __restrict__ stop) {
+    __shared__ T s[kSmall][kSmall + 1];
+    if (stop && *stop) return;
+    load_upper_lds<T>(s, nb, A, lda);
+    __syncthreads();
+    int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= nrows) return;
+    T x[kSmall];
+#pragma unroll
+    for (int j = 0; j < kSmall; ++j) x[j] = (j < nb) ? B[row + (size_t)j * ldb] : T(0);
+#pragma unroll
+    for (int j = 0; j < kSmall; ++j) {
+        if (j < nb) {
+            T acc = x[j];
+#pragma unroll
+            for (int k = 0; k < j; ++k) acc -= x[k] * s[k][j];
+            x[j] = unit ? acc : acc / s[j][j];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kSmall; ++j)
+        if (j < nb) B[row + (size_t)j * ldb] = x[j];
+}
+
+// Unblocked right-looking upper Cholesky of an nb <= 64 diagonal block in LDS.
+// On a non-positive (or NaN) pivot at local step kk, stores base + kk + 1 in
+// *info (first failure wins) and leaves the block partially factored.
+template <typename T>
+__global__ __launch_bounds__(256) void potrf_small_kernel(int nb, T* __restrict__ A, int lda,
+                                                           int* __restrict__ info, int base) {
+    __shared__ T s[kSmall][kSmall + 1];
+    __shared__ int fail;
+    if (*info != 0) return;
+    load_upper_lds<T>(s, nb, A, lda);
+    if (threadIdx.x == 0) fail = 0;
+    __syncthreads();
+    for (int kk = 0; kk < nb; ++kk) {
+        if (threadIdx.x == 0) {
+            T dkk = s[kk][kk];
+            if (!(dkk > T(0)))
+                fail = kk + 1;
+            else
+                s[kk][kk] = sqrt(dkk);
+        }
+        __syncthreads();
+        if (fail) break;
+        T piv = s[kk][kk];
+        for (int j = kk + 1 + threadIdx.x; j < nb; j += blockDim.x) s[kk][j] /= piv;
+        __syncthreads();
+        int w = nb - kk - 1;
+        for (int e = threadIdx.x; e < w * w; e += blockDim.x) {
+            int i = kk + 1 + e % w, j = kk + 1 + e / w;
+            if (i <= j) s[i][j] -= s[kk][i] * s[kk][j];
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
+        int i = e % nb, j = e / nb;
+        if (i <= j) A[i + (size_t)j * lda] = s[i][j];
+    }
+    if (threadIdx.x == 0 && fail) atomicCAS(info, 0, base + fail);
+}
+
+__global__ void set_int_kernel(int* p, int v) { *p = v; }
+
+// --------------------------------------------------------- TRSM drivers
+
+template <typename T>
+static int trsm_rec(dh_handle_t h, char side, char trans, bool unit, int m, int n, const T* A,
+                    int lda, T* B, int ldb, const int* stop) {
+    if (m == 0 || n == 0) return 0;
+    if (side == 'L' && trans == 'T') {
+        // U^T X = B: halve like triangular_solve (blas_like.h:113-122).
+        if (m <= kSmall) {
+            trsm_lut_small_kernel<T><<<dh_cdiv(n, 256), 256, 0, h->stream>>>(m, n, A, lda, B, ldb,
+                                                                              unit, stop);
+            DH_CHECK_LAUNCH();
+            return 0;
+        }
+        int m1 = m / 2, m2 = m - m1;
+        DH_CHECK_STATUS(trsm_rec<T>(h, side, trans, unit, m1, n, A, lda, B, ldb, stop));
+        DH_CHECK_STATUS(gemm_stop<T>(h, true, false, m2, n, m1, T(-1), A + (size_t)m1 * lda, lda,
+                                     B, ldb, T(1), B + m1, ldb, 0, stop));
+        return trsm_rec<T>(h, side, trans, unit, m2, n, A + m1 + (size_t)m1 * lda, lda, B + m1,
+                           ldb, stop);
+    }
+    if (side == 'L') {
+        // U X = B: bottom half first.
+        if (m <= kSmall) {
+            trsm_lun_small_kernel<T><<<dh_cdiv(n, 256), 256, 0, h->stream>>>(m, n, A, lda, B, ldb,
+                                                                              unit, stop);
+            DH_CHECK_LAUNCH();
+            return 0;
+        }
+        int m1 = m / 2, m2 = m - m1;
+        DH_CHECK_STATUS(trsm_rec<T>(h, side, trans, unit, m2, n, A + m1 + (size_t)m1 * lda, lda,
+                                    B + m1, ldb, stop));
+        DH_CHECK_STATUS(gemm_stop<T>(h, false, false, m1, n, m2, T(-1), A + (size_t)m1 * lda,
+                                     lda, B + m1, ldb, T(1), B, ldb, 0, stop));
+        return trsm_rec<T>(h, side, trans, unit, m1, n, A, lda, B, ldb, stop);
+    }
+    // side R, no transpose: X U = B, triangle of order n, B m x n.
+    if (n <= kSmall) {
+        trsm_run_small_kernel<T><<<dh_cdiv(m, 256), 256, 0, h->stream>>>(n, m, A, lda, B, ldb,
+                                                                          unit, stop);
+        DH_CHECK_LAUNCH();
+        return 0;
+    }
+    int n1 = n / 2, n2 = n - n1;
+    DH_CHECK_STATUS(trsm_rec<T>(h, side, trans, unit, m, n1, A, lda, B, ldb, stop));
+    DH_CHECK_STATUS(gemm_stop<T>(h, false, false, m, n2, n1, T(-1), B, ldb, A + (size_t)n1 * lda,
+                                 lda, T(1), B + (size_t)n1 * ldb, ldb, 0, stop));
+    return trsm_rec<T>(h, side, trans, unit, m, n2, A + n1 + (size_t)n1 * lda, lda,
+                       B + (size_t)n1 * ldb, ldb, stop);
+}
+
+template <typename T>
+int dh_trsm_impl(dh_handle_t h, char side, char trans, bool unit, int m, int n, T alpha,
+                 const T* A, int lda, T* B, int ldb) {
+    if (m == 0 || n == 0) return 0;
+    if (alpha != T(1)) {
+        scale_kernel<T><<<dh_cdiv((long)m * n, 256), 256, 0, h->stream>>>(m, n, alpha, B, ldb, 0);
+        DH_CHECK_LAUNCH();
+        if (alpha == T(0)) return 0;
+    }
+    return trsm_rec<T>(h, side, trans, unit, m, n, A, lda, B, ldb, nullptr);
+}
+template int dh_trsm_impl<double>(dh_handle_t, char, char, bool, int, int, double, const double*,
+                                  int, double*, int);
+template int dh_trsm_impl<float>(dh_handle_t, char, char, bool, int, int, float, const float*,
+                                 int, float*, int);
+
+// --------------------------------------------------------- Cholesky driver
+
+// Factor one diagonal block of order b (<= any): right-looking in 64-wide
+// steps entirely on the device; failures land in *dinfo with global index.
+template <typename T>
+static int potrf_block(dh_handle_t h, int b, T* A11, int lda, int* dinfo, int base) {
+    for (int ii = 0; ii < b; ii += kSmall) {
+        int bb = min(kSmall, b - ii);
+        T* Aii = A11 + ii + (size_t)ii * lda;
+        potrf_small_kernel<T><<<1, 256, 0, h->stream>>>(bb, Aii, lda, dinfo, base + ii);
+        DH_CHECK_LAUNCH();
+        if (ii + bb < b) {
+            T* A12 = A11 + ii + (size_t)(ii + bb) * lda;
+            T* A22 = A11 + (ii + bb) + (size_t)(ii + bb) * lda;
+            DH_CHECK_STATUS(trsm_rec<T>(h, 'L', 'T', false, bb, b - ii - bb, Aii, lda, A12, lda,
+                                        dinfo));
+            DH_CHECK_STATUS(gemm_stop<T>(h, true, false, b - ii - bb, b - ii - bb, bb, T(-1), A12,
+                                         lda, A12, lda, T(1), A22, lda, 1, dinfo));
+        }
+    }
+    return 0;
+}
+
+template <typename T>
+int dh_potrf_impl(dh_handle_t h, int n, T* A, int lda, int bs, int* info_host) {
+    *info_host = 0;
+    if (n == 0) return 0;
+    if (bs <= 0) bs = 256;
+    int* dinfo = (int*)dh_ws(h, DH_WS_INFO, 64);
+    if (!dinfo) return DH_ERR_ALLOC;
+    set_int_kernel<<<1, 1, 0, h->stream>>>(dinfo, 0);
+    DH_CHECK_LAUNCH();
+    // cholesky_factorize (lapack_like.h:17-52): potrf(A11); trsm("L","U","T") on
+    // A12; upper SYRK A22 -= A12^T A12.
+    for (int i = 0; i < n; i += bs) {
+        int b = min(bs, n - i);
+        T* A11 = A + i + (size_t)i * lda;
+        DH_CHECK_STATUS(potrf_block<T>(h, b, A11, lda, dinfo, i));
+        if (i + b < n) {
+            T* A12 = A + i + (size_t)(i + b) * lda;
+            T* A22 = A + (i + b) + (size_t)(i + b) * lda;
+            DH_CHECK_STATUS(trsm_rec<T>(h, 'L', 'T', false, b, n - i - b, A11, lda, A12, lda, dinfo));
+            DH_CHECK_STATUS(gemm_stop<T>(h, true, false, n - i - b, n - i - b, b, T(-1), A12, lda,
+                                         A12, lda, T(1), A22, lda, 1, dinfo));
+        }
+    }
+    int hinfo = 0;
+    DH_CHECK_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    DH_CHECK_HIP(hipStreamSynchronize(h->stream));
+    *info_host = hinfo;
+    return 0;
+}
+template int dh_potrf_impl<double>(dh_handle_t, int, double*, int, int, int*);
+template int dh_potrf_impl<float>(dh_handle_t, int, float*, int, int, int*);
+
+// ------------------------------------------------------------- C ABI
+
+template <typename T>
+static int gemm_api(const char* name, dh_handle_t h, char ta, char tb, int m, int n, int k,
+                    T alpha, const T* A, int lda, const T* B, int ldb, T beta, T* C, int ldc) {
+    if (!h) return dh_fail(-1, "%s: NULL handle", name);
+    if (!dh_is_n(ta) && !dh_is_t(ta)) return dh_fail(-2, "%s: bad transa '%c'", name, ta);
+    if (!dh_is_n(tb) && !dh_is_t(tb)) return dh_fail(-3, "%s: bad transb '%c'", name, tb);
+    if (m < 0) return dh_fail(-4, "%s: m < 0", name);
+    if (n < 0) return dh_fail(-5, "%s: n < 0", name);
+    if (k < 0) return dh_fail(-6, "%s: k < 0", name);
+    int arows = dh_is_t(ta) ? k : m, brows = dh_is_t(tb) ? n : k;
+    if (lda < max(1, arows)) return dh_fail(-9, "%s: lda too small", name);
+    if (ldb < max(1, brows)) return dh_fail(-11, "%s: ldb too small", name);
+    if (ldc < max(1, m)) return dh_fail(-14, "%s: ldc too small", name);
+    return dh_gemm_impl<T>(h, dh_is_t(ta), dh_is_t(tb), m, n, k, alpha, A, lda, B, ldb, beta, C,
+                           ldc, 0);
+}
+
+extern "C" int dh_gemm_f64(dh_handle_t h, char ta, char tb, int m, int n, int k, double alpha,
+                           const double* A, int lda, const double* B, int ldb, double beta,
+                           double* C, int ldc) {
+    return gemm_api<double>("dh_gemm_f64", h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                            ldc);
+}
+extern "C" int dh_gemm_f32(dh_handle_t h, char ta, char tb, int m, int n, int k, float alpha,
+                           const float* A, int lda, const float* B, int ldb, float beta, float* C,
+                           int ldc) {
+    return gemm_api<float>("dh_gemm_f32", h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                           ldc);
+}
+
+template <typename T>
+static int syrk_api(const char* name, dh_handle_t h, int n, int k, T alpha, const T* A, int lda,
+                    T beta, T* C, int ldc) {
+    if (!h) return dh_fail(-1, "%s: NULL handle", name);
+    if (n < 0) return dh_fail(-2, "%s: n < 0", name);
+    if (k < 0) return dh_fail(-3, "%s: k < 0", name);
+    if (lda < max(1, k)) return dh_fail(-6, "%s: lda too small", name);
+    if (ldc < max(1, n)) return dh_fail(-9, "%s: ldc too small", name);
+    return dh_gemm_impl<T>(h, true, false, n, n, k, alpha, A, lda, A, lda, beta, C, ldc, 1);
+}
+extern "C" int dh_syrk_ut_f64(dh_handle_t h, int n, int k, double alpha, const double* A, int lda,
+                              double beta, double* C, int ldc) {
+    return syrk_api<double>("dh_syrk_ut_f64", h, n, k, alpha, A, lda, beta, C, ldc);
+}
+extern "C" int dh_syrk_ut_f32(dh_handle_t h, int n, int k, float alpha, const float* A, int lda,
+                              float beta, float* C, int ldc) {
+    return syrk_api<float>("dh_syrk_ut_f32", h, n, k, alpha, A, lda, beta, C, ldc);
+}
+
+template <typename T>
+static int trsm_api(const char* name, dh_handle_t h, char side, char uplo, char trans, char diag,
+                    int m, int n, T alpha, const T* A, int lda, T* B, int ldb) {
+    if (!h) return dh_fail(-1, "%s: NULL handle", name);
+    char s = (side == 'l' ? 'L' : side == 'r' ? 'R' : side);
+    if (s != 'L' && s != 'R') return dh_fail(-2, "%s: bad side '%c'", name, side);
+    if (uplo != 'U' && uplo != 'u')
+        return dh_fail(DH_ERR_UNSUPPORTED, "%s: only uplo='U' is implemented", name);
+    if (!dh_is_n(trans) && !dh_is_t(trans)) return dh_fail(-4, "%s: bad transa '%c'", name, trans);
+    if (diag != 'N' && diag != 'n' && diag != 'U' && diag != 'u')
+        return dh_fail(-5, "%s: bad diag '%c'", name, diag);
+    if (s == 'R' && dh_is_t(trans))
+        return dh_fail(DH_ERR_UNSUPPORTED, "%s: side='R' with transa='T' is not implemented",
+                       name);
+    if (m < 0) return dh_fail(-6, "%s: m < 0", name);
+    if (n < 0) return dh_fail(-7, "%s: n < 0", name);
+    int ka = (s == 'L') ? m : n;
+    if (lda < max(1, ka)) return dh_fail(-10, "%s: lda too small", name);
+    if (ldb < max(1, m)) return dh_fail(-12, "%s: ldb too small", name);
+    return dh_trsm_impl<T>(h, s, dh_is_t(trans) ? 'T' : 'N', diag == 'U' || diag == 'u', m, n,
+                           alpha, A, lda, B, ldb);
+}
+extern "C" int dh_trsm_f64(dh_handle_t h, char side, char uplo, char trans, char diag, int m,
+                           int n, double alpha, const double* A, int lda, double* B, int ldb) {
+    return trsm_api<double>("dh_trsm_f64", h, side, uplo, trans, diag, m, n, alpha, A, lda, B,
+                            ldb);
+}
+extern "C" int dh_trsm_f32(dh_handle_t h, char side, char uplo, char trans, char diag, int m,
+                           int n, float alpha, const float* A, int lda, float* B, int ldb) {
+    return trsm_api<float>("dh_trsm_f32", h, side, uplo, trans, diag, m, n, alpha, A, lda, B,
+                           ldb);
+}
+
+template <typename T>
+static int potrf_api(const char* name, dh_handle_t h, char uplo, int n, T* A, int lda, int bs) {
+    if (!h) return dh_fail(-1, "%s: NULL handle", name);
+    if (uplo != 'U' && uplo != 'u')
+        return dh_fail(DH_ERR_UNSUPPORTED, "%s: only uplo='U' is implemented", name);
+    if (n < 0) return dh_fail(-3, "%s: n < 0", name);
+    if (lda < max(1, n)) return dh_fail(-5, "%s: lda too small", name);
+    int info = 0;
+    DH_CHECK_STATUS(dh_potrf_impl<T>(h, n, A, lda, bs, &info));
+    if (info) dh_fail(info, "%s: leading minor of order %d is not positive definite", name, info);
+    return info;
+}
+extern "C" int dh_potrf_f64(dh_handle_t h, char uplo, int n, double* A, int lda, int bs) {
+    return potrf_api<double>("dh_potrf_f64", h, uplo, n, A, lda, bs);
+}
+extern "C" int dh_potrf_f32(dh_handle_t h, char uplo, int n, float* A, int lda, int bs) {
+    return potrf_api<float>("dh_potrf_f32", h, uplo, n, A, lda, bs);
+}

@@ -1,0 +1,71 @@
+// Internal declarations shared by the HIP sources of libdarray_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/darray_hip.h"
+
+// Workspace slots owned by a handle.  Each slot is one device buffer that is
+// grown on demand and reused by later calls; two calls that are live at the
+// same time never share a slot.
+enum dh_slot {
+    DH_WS_SPLITK = 0,  // split-K partial products
+    DH_WS_NORMS,       // squared norms of point sets
+    DH_WS_GRAM,        // CholeskyQR Gram matrix / R factors
+    DH_WS_RBUF,        // second R factor
+    DH_WS_YBUF,        // n x l panel buffer of the range finder
+    DH_WS_LOWP_X,      // low-precision copy of X
+    DH_WS_LOWP_O,      // low-precision packed Omega / Q
+    DH_WS_LOWP_Y,      // FP32 output of a low-precision pass
+    DH_WS_PARTIAL,     // per-workgroup partial sums
+    DH_WS_INFO,        // device status words
+    DH_WS_COUNT
+};
+
+struct dh_handle_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    void* ws[DH_WS_COUNT] = {};
+    size_t ws_size[DH_WS_COUNT] = {};
+    int num_cu = 0;
+};
+
+// Records a formatted message for dh_last_error_string and returns `code`.
+int dh_fail(int code, const char* fmt, ...);
+
+// Device buffer of at least `bytes` in `slot`; nullptr on failure (message set).
+void* dh_ws(dh_handle_t h, dh_slot slot, size_t bytes);
+
+#define DH_CHECK_HIP(expr)                                                            \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return dh_fail(DH_ERR_HIP, "%s:%d: %s failed: %s", __FILE__, __LINE__, #expr, \
+                           hipGetErrorString(e_));                                    \
+    } while (0)
+
+#define DH_CHECK_LAUNCH() DH_CHECK_HIP(hipGetLastError())
+
+#define DH_CHECK_STATUS(expr)  \
+    do {                       \
+        int s_ = (expr);       \
+        if (s_ != 0) return s_; \
+    } while (0)
+
+static inline bool dh_is_t(char c) { return c == 'T' || c == 't' || c == 'C' || c == 'c'; }
+static inline bool dh_is_n(char c) { return c == 'N' || c == 'n'; }
+static inline int dh_cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+// Dense building blocks used across translation units (dense.hip).
+template <typename T>
+int dh_gemm_impl(dh_handle_t h, bool ta, bool tb, int m, int n, int k, T alpha, const T* A,
+                 int lda, const T* B, int ldb, T beta, T* C, int ldc, int uplo_upper);
+template <typename T>
+int dh_trsm_impl(dh_handle_t h, char side, char trans, bool unit, int m, int n, T alpha,
+                 const T* A, int lda, T* B, int ldb);
+template <typename T>
+int dh_potrf_impl(dh_handle_t h, int n, T* A, int lda, int bs, int* info_host);

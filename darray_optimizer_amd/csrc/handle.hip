@@ -1,0 +1,104 @@
+// Handle, stream and workspace management; error reporting.
+//
+// Replaces cuda_init / cuda_finalize (dkernel.cu:28-39), which created process
+// globals (one cuBLAS handle, one cuSOLVER handle, one stream) and launched
+// the custom kernels on the legacy default stream.  Here every call runs on
+// the handle's one stream, and device workspaces are allocated once and
+// grown on demand instead of the per-call cudaMallocManaged / cudaFree of
+// rbf_kernel and tc_gemm (dkernel.cu:135-147, 196-198).
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "dh_common.h"
+
+static thread_local char g_last_error[1024] = "";
+
+int dh_fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+extern "C" const char* dh_last_error_string(void) { return g_last_error; }
+
+void* dh_ws(dh_handle_t h, dh_slot slot, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    if (h->ws_size[slot] >= bytes) return h->ws[slot];
+    // Grow: the old buffer may still be read by queued work.
+    if (h->ws[slot]) {
+        if (hipStreamSynchronize(h->stream) != hipSuccess || hipFree(h->ws[slot]) != hipSuccess) {
+            dh_fail(DH_ERR_HIP, "workspace release failed");
+            return nullptr;
+        }
+        h->ws[slot] = nullptr;
+        h->ws_size[slot] = 0;
+    }
+    size_t want = bytes + bytes / 8;  // headroom for slightly larger later calls
+    want = (want + 4095) & ~size_t(4095);
+    void* p = nullptr;
+    if (hipMalloc(&p, want) != hipSuccess) {
+        (void)hipGetLastError();
+        dh_fail(DH_ERR_ALLOC, "hipMalloc of %zu bytes (workspace slot %d) failed", want, (int)slot);
+        return nullptr;
+    }
+    h->ws[slot] = p;
+    h->ws_size[slot] = want;
+    return p;
+}
+
+extern "C" int dh_create(int device, void* stream, dh_handle_t* out) {
+    if (!out) return dh_fail(-3, "dh_create: out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    DH_CHECK_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev)
+        return dh_fail(-1, "dh_create: device %d out of range (%d devices)", device, ndev);
+    DH_CHECK_HIP(hipSetDevice(device));
+    dh_handle_s* h = new (std::nothrow) dh_handle_s();
+    if (!h) return dh_fail(DH_ERR_ALLOC, "dh_create: out of host memory");
+    h->device = device;
+    if (stream) {
+        h->stream = (hipStream_t)stream;
+        h->own_stream = false;
+    } else {
+        hipError_t e = hipStreamCreate(&h->stream);  // default priority
+        if (e != hipSuccess) {
+            delete h;
+            return dh_fail(DH_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+        }
+        h->own_stream = true;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) h->num_cu = prop.multiProcessorCount;
+    if (h->num_cu <= 0) h->num_cu = 256;
+    *out = h;
+    return 0;
+}
+
+extern "C" int dh_destroy(dh_handle_t h) {
+    if (!h) return 0;
+    (void)hipSetDevice(h->device);
+    (void)hipStreamSynchronize(h->stream);
+    for (int s = 0; s < DH_WS_COUNT; ++s)
+        if (h->ws[s]) (void)hipFree(h->ws[s]);
+    if (h->own_stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return 0;
+}
+
+extern "C" int dh_synchronize(dh_handle_t h) {
+    if (!h) return dh_fail(-1, "dh_synchronize: NULL handle");
+    DH_CHECK_HIP(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" size_t dh_workspace_bytes(dh_handle_t h) {
+    if (!h) return 0;
+    size_t s = 0;
+    for (int i = 0; i < DH_WS_COUNT; ++i) s += h->ws_size[i];
+    return s;
+}

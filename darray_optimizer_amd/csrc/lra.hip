@@ -1,0 +1,131 @@
+// CholeskyQR2 and the randomised range finder (DArray::LRA + the
+// orthogonalisation and power iterations around it).
+//
+// Reference pipeline (testsvm.cpp:86-122): LRA (K*Omega, dkernel.h:12-46) ->
+// tall_skinny_qr_factorize (Householder TSQR, lapack_like.h:281-379) -> LRA
+// again for K*Q.  Here the orthogonalisation is CholeskyQR2 (Gram on MFMA,
+// small blocked Cholesky, right triangular solve, twice) and the K-passes
+// never materialise K.
+#include "dh_common.h"
+
+int dh_project_f64_impl(dh_handle_t h, int m, int n, int d, int l, const double* Xi, int ldxi,
+                        const double* Xj, int ldxj, const double* ni, const double* nj,
+                        const double* yi, const double* yj, double gamma, const double* O,
+                        int ldo, double beta, double* Y, int ldy);
+int dh_lra_bf16_passes(dh_handle_t h, int n, int d, int l, int q, const double* X, int ldx,
+                       const double* y, double gamma, const double* Omega, int ldo, double* Q,
+                       int ldq, double* W, int ldw);
+
+// dst (n x n, ld n) = triu(src); strict lower set to zero.
+__global__ __launch_bounds__(256) void triu_copy_kernel(int n, const double* __restrict__ src,
+                                                         int lds, double* __restrict__ dst) {
+    size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (size_t)n * n) return;
+    int i = (int)(idx % n), j = (int)(idx / n);
+    dst[idx] = (i <= j) ? src[i + (size_t)j * lds] : 0.0;
+}
+
+static int cholqr_sweep(dh_handle_t h, int m, int n, double* Y, int ldy, double* G, int* info) {
+    // G = upper(Y^T Y) (SYRK, split over m), G = U^T U, Y := Y U^{-1}.
+    DH_CHECK_STATUS(dh_gemm_impl<double>(h, true, false, n, n, m, 1.0, Y, ldy, Y, ldy, 0.0, G, n, 1));
+    DH_CHECK_STATUS(dh_potrf_impl<double>(h, n, G, n, 64, info));
+    if (*info) return 0;
+    return dh_trsm_impl<double>(h, 'R', 'N', false, m, n, 1.0, G, n, Y, ldy);
+}
+
+int dh_cholqr2_impl(dh_handle_t h, int m, int n, double* Y, int ldy, double* R, int ldr) {
+    if (n == 0) return 0;
+    size_t nn = (size_t)n * n;
+    double* G = (double*)dh_ws(h, DH_WS_GRAM, sizeof(double) * nn * 2);
+    if (!G) return DH_ERR_ALLOC;
+    double* R1 = G + nn;
+    int info = 0;
+    DH_CHECK_STATUS(cholqr_sweep(h, m, n, Y, ldy, G, &info));
+    if (info) return dh_fail(info, "CholeskyQR2: Gram matrix of sweep 1 not positive definite (minor %d)", info);
+    if (R) {
+        triu_copy_kernel<<<dh_cdiv((long)nn, 256), 256, 0, h->stream>>>(n, G, n, R1);
+        DH_CHECK_LAUNCH();
+    }
+    DH_CHECK_STATUS(cholqr_sweep(h, m, n, Y, ldy, G, &info));
+    if (info)
+        return dh_fail(1000000 + info,
+                       "CholeskyQR2: Gram matrix of sweep 2 not positive definite (minor %d)", info);
+    if (R) {
+        // R = R2 * R1 (both upper): the lower triangle of G still holds Gram
+        // entries, so copy its upper part out first.
+        double* R2 = (double*)dh_ws(h, DH_WS_RBUF, sizeof(double) * nn);
+        if (!R2) return DH_ERR_ALLOC;
+        triu_copy_kernel<<<dh_cdiv((long)nn, 256), 256, 0, h->stream>>>(n, G, n, R2);
+        DH_CHECK_LAUNCH();
+        DH_CHECK_STATUS(
+            dh_gemm_impl<double>(h, false, false, n, n, n, 1.0, R2, n, R1, n, 0.0, R, ldr, 0));
+    }
+    return 0;
+}
+
+extern "C" int dh_cholqr2_f64(dh_handle_t h, int m, int n, double* Y, int ldy, double* R,
+                              int ldr) {
+    const char* name = "dh_cholqr2_f64";
+    if (!h) return dh_fail(-1, "%s: NULL handle", name);
+    if (m < 0) return dh_fail(-2, "%s: m < 0", name);
+    if (n < 0 || n > m) return dh_fail(-3, "%s: need 0 <= n <= m", name);
+    if (ldy < max(1, m)) return dh_fail(-5, "%s: ldy < m", name);
+    if (R && ldr < max(1, n)) return dh_fail(-7, "%s: ldr < n", name);
+    int st = dh_cholqr2_impl(h, m, n, Y, ldy, R, ldr);
+    if (st) return st;
+    DH_CHECK_HIP(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+static int copy_panel(dh_handle_t h, int n, int l, const double* src, int lds, double* dst,
+                      int ldd) {
+    DH_CHECK_HIP(hipMemcpy2DAsync(dst, sizeof(double) * (size_t)ldd, src,
+                                  sizeof(double) * (size_t)lds, sizeof(double) * (size_t)n, l,
+                                  hipMemcpyDeviceToDevice, h->stream));
+    return 0;
+}
+
+// FP64 passes: Y = K*Omega -> Q; q x { K*Q -> Q }; W = K*Q.
+static int lra_f64(dh_handle_t h, int n, int d, int l, int q, const double* X, int ldx,
+                   const double* y, double gamma, const double* Omega, int ldo, double* Q,
+                   int ldq, double* W, int ldw) {
+    double* nrm = (double*)dh_ws(h, DH_WS_NORMS, sizeof(double) * (size_t)n);
+    if (!nrm) return DH_ERR_ALLOC;
+    DH_CHECK_STATUS(dh_sqnorms_f64(h, n, d, X, ldx, nrm));
+    DH_CHECK_STATUS(dh_project_f64_impl(h, n, n, d, l, X, ldx, X, ldx, nrm, nrm, y, y, gamma,
+                                        Omega, ldo, 0.0, Q, ldq));
+    DH_CHECK_STATUS(dh_cholqr2_impl(h, n, l, Q, ldq, nullptr, 0));
+    for (int it = 0; it < q; ++it) {
+        DH_CHECK_STATUS(dh_project_f64_impl(h, n, n, d, l, X, ldx, X, ldx, nrm, nrm, y, y, gamma,
+                                            Q, ldq, 0.0, W, ldw));
+        DH_CHECK_STATUS(dh_cholqr2_impl(h, n, l, W, ldw, nullptr, 0));
+        DH_CHECK_STATUS(copy_panel(h, n, l, W, ldw, Q, ldq));
+    }
+    return dh_project_f64_impl(h, n, n, d, l, X, ldx, X, ldx, nrm, nrm, y, y, gamma, Q, ldq, 0.0,
+                               W, ldw);
+}
+
+extern "C" int dh_lra(dh_handle_t h, int prec, int n, int d, int l, int q, const double* X,
+                      int ldx, const double* y, double gamma, const double* Omega, int ldo,
+                      double* Q, int ldq, double* W, int ldw) {
+    const char* name = "dh_lra";
+    if (!h) return dh_fail(-1, "%s: NULL handle", name);
+    if (prec != DH_PREC_F64 && prec != DH_PREC_BF16) return dh_fail(-2, "%s: bad prec %d", name, prec);
+    if (n < 0) return dh_fail(-3, "%s: n < 0", name);
+    if (d < 1) return dh_fail(-4, "%s: d < 1", name);
+    if (l < 0 || l > n) return dh_fail(-5, "%s: need 0 <= l <= n", name);
+    if (q < 0) return dh_fail(-6, "%s: q < 0", name);
+    if (ldx < d) return dh_fail(-8, "%s: ldx < d", name);
+    if (ldo < max(1, n)) return dh_fail(-12, "%s: ldo < n", name);
+    if (ldq < max(1, n)) return dh_fail(-14, "%s: ldq < n", name);
+    if (ldw < max(1, n)) return dh_fail(-16, "%s: ldw < n", name);
+    if (n == 0 || l == 0) return 0;
+    int st;
+    if (prec == DH_PREC_F64)
+        st = lra_f64(h, n, d, l, q, X, ldx, y, gamma, Omega, ldo, Q, ldq, W, ldw);
+    else
+        st = dh_lra_bf16_passes(h, n, d, l, q, X, ldx, y, gamma, Omega, ldo, Q, ldq, W, ldw);
+    if (st) return st;
+    DH_CHECK_HIP(hipStreamSynchronize(h->stream));
+    return 0;
+}

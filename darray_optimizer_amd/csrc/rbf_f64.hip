@@ -1,0 +1,433 @@
+// RBF kernel on gfx950: squared norms, materialised tiles, the fused FP64
+// K*Omega projection, and the FP64 error pass.
+//
+// Reference: rbf_kernel (dkernel.cu:132-190) = vecnorm (102-114) + distance
+// GEMM (160-169) + elementwise rbf (116-129), with the K tile copied back to
+// the host, replicated, uploaded again and multiplied by Omega in tc_gemm
+// (dkernel.h:33-42, dkernel.cu:194-231).  Here the distance GEMM and the
+// projection are two MFMA stages of one kernel and K stays in registers.
+#include "dh_common.h"
+#include "mfma_tile.h"
+
+// ------------------------------------------------------------------ norms
+
+template <typename T>
+__global__ __launch_bounds__(256) void sqnorms_kernel(int n, int d, const T* __restrict__ X,
+                                                       int ldx, T* __restrict__ nrm) {
+    // One wave per point: lanes stride the (contiguous) features, then a
+    // butterfly sum.  Summation order differs from the sequential vecnorm.
+    int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    if (wave >= n) return;
+    const T* x = X + (size_t)wave * ldx;
+    T s = 0;
+    for (int f = lane; f < d; f += 64) s += x[f] * x[f];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) nrm[wave] = s;
+}
+
+template <typename T>
+static int launch_sqnorms(dh_handle_t h, int n, int d, const T* X, int ldx, T* nrm) {
+    if (n == 0) return 0;
+    sqnorms_kernel<T><<<dh_cdiv(n, 4), 256, 0, h->stream>>>(n, d, X, ldx, nrm);
+    DH_CHECK_LAUNCH();
+    return 0;
+}
+
+extern "C" int dh_sqnorms_f64(dh_handle_t h, int n, int d, const double* X, int ldx,
+                              double* nrm) {
+    if (!h) return dh_fail(-1, "dh_sqnorms_f64: NULL handle");
+    if (n < 0) return dh_fail(-2, "dh_sqnorms_f64: n < 0");
+    if (d < 0) return dh_fail(-3, "dh_sqnorms_f64: d < 0");
+    if (ldx < max(1, d)) return dh_fail(-5, "dh_sqnorms_f64: ldx < d");
+    return launch_sqnorms<double>(h, n, d, X, ldx, nrm);
+}
+
+template <typename T>
+__device__ __forceinline__ T rbf_exp(T x);
+template <>
+__device__ __forceinline__ double rbf_exp<double>(double x) {
+    return exp(x);
+}
+template <>
+__device__ __forceinline__ float rbf_exp<float>(float x) {
+    return __expf(x);  // the reference's fast exp (dkernel.cu:125)
+}
+
+// ------------------------------------------------------- materialised tile
+//
+// Distance GEMM op(A) = Xi^T (point-major, contiguous features), op(B) = Xj,
+// then the rbf epilogue on the accumulators: K = yi*yj*exp(-gamma*max(d2, 0)).
+template <typename T, int BM, int BN>
+__global__ __launch_bounds__(256) void rbf_tile_kernel(int m, int n, int d,
+                                                        const T* __restrict__ Xi, int ldxi,
+                                                        const T* __restrict__ Xj, int ldxj,
+                                                        const T* __restrict__ ni,
+                                                        const T* __restrict__ nj,
+                                                        const T* __restrict__ yi,
+                                                        const T* __restrict__ yj, T gamma,
+                                                        T* __restrict__ K, int ldk) {
+    constexpr int FM = BM / 32, FN = BN / 32;
+    __shared__ T sA[kTileBK][BM + kTilePad];
+    __shared__ T sB[kTileBK][BN + kTilePad];
+    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    typename Mfma<T>::acc_t acc[FM][FN];
+    zero_acc<T, FM, FN>(acc);
+    tile_mma<T, BM, BN, true, false>(acc, sA, sB, m, n, m0, n0, 0, d, Xi, ldxi, Xj, ldxj);
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int i = m0 + tile_row<BM, BN>(a, r);
+                int j = n0 + tile_col<BM, BN>(b);
+                if (i < m && j < n) {
+                    T d2 = ni[i] + nj[j] - T(2) * acc[a][b][r];
+                    d2 = d2 > T(0) ? d2 : T(0);
+                    T v = rbf_exp<T>(-gamma * d2);
+                    if (yi) v *= yi[i];
+                    if (yj) v *= yj[j];
+                    K[i + (size_t)j * ldk] = v;
+                }
+            }
+}
+
+template <typename T>
+static int rbf_tile_api(const char* name, dh_handle_t h, int m, int n, int d, const T* Xi,
+                        int ldxi, const T* Xj, int ldxj, const T* yi, const T* yj, T gamma, T* K,
+                        int ldk) {
+    if (!h) return dh_fail(-1, "%s: NULL handle", name);
+    if (m < 0) return dh_fail(-2, "%s: m < 0", name);
+    if (n < 0) return dh_fail(-3, "%s: n < 0", name);
+    if (d < 0) return dh_fail(-4, "%s: d < 0", name);
+    if (ldxi < max(1, d)) return dh_fail(-6, "%s: ldxi < d", name);
+    if (ldxj < max(1, d)) return dh_fail(-8, "%s: ldxj < d", name);
+    if (ldk < max(1, m)) return dh_fail(-13, "%s: ldk < m", name);
+    if (m == 0 || n == 0) return 0;
+    T* nrm = (T*)dh_ws(h, DH_WS_NORMS, sizeof(T) * ((size_t)m + n));
+    if (!nrm) return DH_ERR_ALLOC;
+    DH_CHECK_STATUS(launch_sqnorms<T>(h, m, d, Xi, ldxi, nrm));
+    DH_CHECK_STATUS(launch_sqnorms<T>(h, n, d, Xj, ldxj, nrm + m));
+    dim3 grid(dh_cdiv(m, 64), dh_cdiv(n, 64));
+    rbf_tile_kernel<T, 64, 64><<<grid, 256, 0, h->stream>>>(m, n, d, Xi, ldxi, Xj, ldxj, nrm,
+                                                             nrm + m, yi, yj, gamma, K, ldk);
+    DH_CHECK_LAUNCH();
+    return 0;
+}
+
+extern "C" int dh_rbf_tile_f64(dh_handle_t h, int m, int n, int d, const double* Xi, int ldxi,
+                               const double* Xj, int ldxj, const double* yi, const double* yj,
+                               double gamma, double* K, int ldk) {
+    return rbf_tile_api<double>("dh_rbf_tile_f64", h, m, n, d, Xi, ldxi, Xj, ldxj, yi, yj, gamma,
+                                K, ldk);
+}
+extern "C" int dh_rbf_tile_f32(dh_handle_t h, int m, int n, int d, const float* Xi, int ldxi,
+                               const float* Xj, int ldxj, const float* yi, const float* yj,
+                               float gamma, float* K, int ldk) {
+    return rbf_tile_api<float>("dh_rbf_tile_f32", h, m, n, d, Xi, ldxi, Xj, ldxj, yi, yj, gamma, K,
+                               ldk);
+}
+
+// ------------------------------------------------- fused FP64 K * Omega
+//
+// Workgroup: 4 waves, 64 rows i (16 per wave).  The j loop walks Xj and
+// Omega in chunks of 16 points staged in LDS.  Per chunk each wave
+//   1. forms S^T = Xj_chunk * Xi_wave^T (16 j x 16 i) with DK f64 MFMAs,
+//      A operand from LDS (Xj^T, f-major), B operand from registers (Xi);
+//   2. turns its 4 accumulators into K values in place (exp epilogue);
+//   3. uses them directly as the A operand of the projection MFMAs: lane L
+//      holds S^T[4*(L/16)+r][L%16] = K[i=L%16][j=4*(L/16)+r], i.e. MFMA r
+//      contracts over j = {r, 4+r, 8+r, 12+r}; Omega is staged in LDS in the
+//      matching order sO[r][cb][L] = Omega[j0+4*(L/16)+r][c0+16*cb+L%16].
+// Labels are folded: yj scales Omega as it is staged, yi scales the output.
+// The projection accumulators (NCB blocks of 16 columns) stay in registers
+// for the whole j loop; Y is written once.
+template <int DK, int NCB>
+__global__ __launch_bounds__(256) void rbf_project_f64_kernel(
+    int m, int n, int d, int l, const double* __restrict__ Xi, int ldxi,
+    const double* __restrict__ Xj, int ldxj, const double* __restrict__ ni,
+    const double* __restrict__ nj, const double* __restrict__ yi, const double* __restrict__ yj,
+    double gamma, const double* __restrict__ O, int ldo, double beta, double* __restrict__ Y,
+    int ldy) {
+    constexpr int JB = 16;
+    constexpr int FD = DK * 4;  // padded feature count
+    __shared__ double sX[FD][JB];
+    __shared__ double sO[4][NCB][64];
+    __shared__ double sN[JB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane >> 4, li = lane & 15;
+    const int c0 = blockIdx.y * NCB * 16;
+    const int rbase = blockIdx.x * 64 + wave * 16;
+    const int row = rbase + li;
+
+    double xr[DK];
+#pragma unroll
+    for (int kk = 0; kk < DK; ++kk) {
+        int f = 4 * kk + g;
+        xr[kk] = (row < m && f < d) ? Xi[f + (size_t)row * ldxi] : 0.0;
+    }
+    const double nrow = (row < m) ? ni[row] : 0.0;
+    dh_d4 acc[NCB];
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) acc[cb] = dh_d4{0, 0, 0, 0};
+
+    for (int j0 = 0; j0 < n; j0 += JB) {
+        __syncthreads();
+        for (int e = tid; e < JB * FD; e += 256) {
+            int jj = e / FD, f = e % FD, j = j0 + jj;
+            sX[f][jj] = (j < n && f < d) ? Xj[f + (size_t)j * ldxj] : 0.0;
+        }
+        for (int e = tid; e < JB * NCB * 16; e += 256) {
+            int jj = e % JB, cl = e / JB, j = j0 + jj, c = c0 + cl;
+            double v = 0.0;
+            if (j < n && c < l) {
+                v = O[j + (size_t)c * ldo];
+                if (yj) v *= yj[j];
+            }
+            sO[jj & 3][cl >> 4][(jj >> 2) * 16 + (cl & 15)] = v;
+        }
+        if (tid < JB) sN[tid] = (j0 + tid < n) ? nj[j0 + tid] : 0.0;
+        __syncthreads();
+
+        dh_d4 s = dh_d4{0, 0, 0, 0};
+#pragma unroll
+        for (int kk = 0; kk < DK; ++kk)
+            s = __builtin_amdgcn_mfma_f64_16x16x4f64(sX[4 * kk + g][li], xr[kk], s, 0, 0, 0);
+        double kv[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            double d2 = nrow + sN[4 * g + r] - 2.0 * s[r];
+            d2 = d2 > 0.0 ? d2 : 0.0;
+            kv[r] = exp(-gamma * d2);
+        }
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(kv[r], sO[r][cb][lane], acc[cb], 0,
+                                                               0, 0);
+    }
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int i = rbase + 4 * g + r, c = c0 + 16 * cb + li;
+            if (i < m && c < l) {
+                double v = acc[cb][r];
+                if (yi) v *= yi[i];
+                double* yp = Y + i + (size_t)c * ldy;
+                *yp = (beta == 0.0) ? v : v + beta * (*yp);
+            }
+        }
+}
+
+template <int DK, int NCB>
+static void launch_project_f64(dh_handle_t h, dim3 grid, int m, int n, int d, int l,
+                               const double* Xi, int ldxi, const double* Xj, int ldxj,
+                               const double* ni, const double* nj, const double* yi,
+                               const double* yj, double gamma, const double* O, int ldo,
+                               double beta, double* Y, int ldy) {
+    rbf_project_f64_kernel<DK, NCB><<<grid, 256, 0, h->stream>>>(
+        m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, O, ldo, beta, Y, ldy);
+}
+
+template <int DK>
+static int dispatch_ncb(dh_handle_t h, int ncb, dim3 grid, int m, int n, int d, int l,
+                        const double* Xi, int ldxi, const double* Xj, int ldxj, const double* ni,
+                        const double* nj, const double* yi, const double* yj, double gamma,
+                        const double* O, int ldo, double beta, double* Y, int ldy) {
+#define DH_NCB(N)                                                                              \
+    case N:                                                                                    \
+        launch_project_f64<DK, N>(h, grid, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, \
+                                  O, ldo, beta, Y, ldy);                                       \
+        break;
+    switch (ncb) {
+        DH_NCB(1)
+        DH_NCB(2)
+        DH_NCB(3)
+        DH_NCB(4)
+        DH_NCB(5)
+        DH_NCB(6)
+        DH_NCB(8)
+        DH_NCB(9)
+        DH_NCB(12)
+        DH_NCB(17)
+        default:
+            return dh_fail(DH_ERR_UNSUPPORTED, "rbf_project_f64: no kernel for %d column blocks",
+                           ncb);
+    }
+#undef DH_NCB
+    DH_CHECK_LAUNCH();
+    return 0;
+}
+
+// Smallest instantiated column-block count >= want.
+static int ncb_instance(int want) {
+    static const int inst[] = {1, 2, 3, 4, 5, 6, 8, 9, 12, 17};
+    for (int v : inst)
+        if (v >= want) return v;
+    return 17;
+}
+
+int dh_project_f64_impl(dh_handle_t h, int m, int n, int d, int l, const double* Xi, int ldxi,
+                        const double* Xj, int ldxj, const double* ni, const double* nj,
+                        const double* yi, const double* yj, double gamma, const double* O,
+                        int ldo, double beta, double* Y, int ldy) {
+    if (m == 0 || l == 0) return 0;
+    int ncb_total = dh_cdiv(l, 16);
+    int groups = dh_cdiv(ncb_total, 17);
+    int ncb = ncb_instance(dh_cdiv(ncb_total, groups));
+    groups = dh_cdiv(ncb_total, ncb);
+    dim3 grid(dh_cdiv(m, 64), groups);
+    if (d <= 32)
+        return dispatch_ncb<8>(h, ncb, grid, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma,
+                               O, ldo, beta, Y, ldy);
+    if (d <= 64)
+        return dispatch_ncb<16>(h, ncb, grid, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj,
+                                gamma, O, ldo, beta, Y, ldy);
+    if (d <= 128)
+        return dispatch_ncb<32>(h, ncb, grid, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj,
+                                gamma, O, ldo, beta, Y, ldy);
+    if (d <= 256)
+        return dispatch_ncb<64>(h, ncb, grid, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj,
+                                gamma, O, ldo, beta, Y, ldy);
+    return dh_fail(DH_ERR_UNSUPPORTED, "rbf_project_f64: d = %d > 256", d);
+}
+
+extern "C" int dh_rbf_project_f64(dh_handle_t h, int m, int n, int d, int l, const double* Xi,
+                                  int ldxi, const double* Xj, int ldxj, const double* yi,
+                                  const double* yj, double gamma, const double* O, int ldo,
+                                  double beta, double* Y, int ldy) {
+    const char* name = "dh_rbf_project_f64";
+    if (!h) return dh_fail(-1, "%s: NULL handle", name);
+    if (m < 0) return dh_fail(-2, "%s: m < 0", name);
+    if (n < 0) return dh_fail(-3, "%s: n < 0", name);
+    if (d < 0) return dh_fail(-4, "%s: d < 0", name);
+    if (l < 0) return dh_fail(-5, "%s: l < 0", name);
+    if (ldxi < max(1, d)) return dh_fail(-7, "%s: ldxi < d", name);
+    if (ldxj < max(1, d)) return dh_fail(-9, "%s: ldxj < d", name);
+    if (ldo < max(1, n)) return dh_fail(-14, "%s: ldo < n", name);
+    if (ldy < max(1, m)) return dh_fail(-17, "%s: ldy < m", name);
+    if (m == 0 || l == 0) return 0;
+    double* nrm = (double*)dh_ws(h, DH_WS_NORMS, sizeof(double) * ((size_t)m + n + 1));
+    if (!nrm) return DH_ERR_ALLOC;
+    DH_CHECK_STATUS(launch_sqnorms<double>(h, m, d, Xi, ldxi, nrm));
+    DH_CHECK_STATUS(launch_sqnorms<double>(h, n, d, Xj, ldxj, nrm + m));
+    return dh_project_f64_impl(h, m, n, d, l, Xi, ldxi, Xj, ldxj, nrm, nrm + m, yi, yj, gamma, O,
+                               ldo, beta, Y, ldy);
+}
+
+// ------------------------------------------------------------ error pass
+//
+// Per 64 x 64 tile: S = Xi^T Xj (k over d) and P = Q_i W_j^T (k over l) in two
+// MFMA accumulator sets, then sum (K - P)^2 and K^2 in FP64.  One pair of
+// partial sums per workgroup, reduced in a fixed order by a second kernel.
+__global__ __launch_bounds__(256) void lra_error_f64_kernel(
+    int n, int d, int l, const double* __restrict__ X, int ldx, const double* __restrict__ nrm,
+    const double* __restrict__ y, double gamma, const double* __restrict__ Q, int ldq,
+    const double* __restrict__ W, int ldw, double* __restrict__ partial) {
+    constexpr int BM = 64, BN = 64, FM = 2, FN = 2;
+    __shared__ double sA[kTileBK][BM + kTilePad];
+    __shared__ double sB[kTileBK][BN + kTilePad];
+    __shared__ double red[2][4];
+    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    dh_d4 S[FM][FN], P[FM][FN];
+    zero_acc<double, FM, FN>(S);
+    zero_acc<double, FM, FN>(P);
+    tile_mma<double, BM, BN, true, false>(S, sA, sB, n, n, m0, n0, 0, d, X, ldx, X, ldx);
+    tile_mma<double, BM, BN, false, true>(P, sA, sB, n, n, m0, n0, 0, l, Q, ldq, W, ldw);
+    double se = 0.0, sk = 0.0;
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int i = m0 + tile_row<BM, BN>(a, r);
+                int j = n0 + tile_col<BM, BN>(b);
+                if (i < n && j < n) {
+                    double d2 = nrm[i] + nrm[j] - 2.0 * S[a][b][r];
+                    d2 = d2 > 0.0 ? d2 : 0.0;
+                    double kv = exp(-gamma * d2);
+                    if (y) kv *= y[i] * y[j];
+                    double e = kv - P[a][b][r];
+                    se += e * e;
+                    sk += kv * kv;
+                }
+            }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        se += __shfl_xor(se, o);
+        sk += __shfl_xor(sk, o);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][wave] = se;
+        red[1][wave] = sk;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        size_t b = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+        partial[2 * b] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+        partial[2 * b + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+    }
+}
+
+// Fixed-order reduction of `count` (se, sk) pairs into out[0..1].
+__global__ __launch_bounds__(256) void reduce_pairs_kernel(size_t count,
+                                                            const double* __restrict__ partial,
+                                                            double* __restrict__ out) {
+    __shared__ double red[2][256];
+    double se = 0.0, sk = 0.0;
+    for (size_t b = threadIdx.x; b < count; b += 256) {
+        se += partial[2 * b];
+        sk += partial[2 * b + 1];
+    }
+    red[0][threadIdx.x] = se;
+    red[1][threadIdx.x] = sk;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = red[0][0];
+        out[1] = red[1][0];
+    }
+}
+
+extern "C" int dh_lra_error(dh_handle_t h, int n, int d, int l, const double* X, int ldx,
+                            const double* y, double gamma, const double* Q, int ldq,
+                            const double* W, int ldw, double* err2) {
+    const char* name = "dh_lra_error";
+    if (!h) return dh_fail(-1, "%s: NULL handle", name);
+    if (n < 0) return dh_fail(-2, "%s: n < 0", name);
+    if (d < 0) return dh_fail(-3, "%s: d < 0", name);
+    if (l < 0) return dh_fail(-4, "%s: l < 0", name);
+    if (ldx < max(1, d)) return dh_fail(-6, "%s: ldx < d", name);
+    if (ldq < max(1, n)) return dh_fail(-10, "%s: ldq < n", name);
+    if (ldw < max(1, n)) return dh_fail(-12, "%s: ldw < n", name);
+    if (!err2) return dh_fail(-13, "%s: err2 is NULL", name);
+    err2[0] = err2[1] = 0.0;
+    if (n == 0) return 0;
+    double* nrm = (double*)dh_ws(h, DH_WS_NORMS, sizeof(double) * (size_t)n);
+    if (!nrm) return DH_ERR_ALLOC;
+    DH_CHECK_STATUS(launch_sqnorms<double>(h, n, d, X, ldx, nrm));
+    dim3 grid(dh_cdiv(n, 64), dh_cdiv(n, 64));
+    size_t nblk = (size_t)grid.x * grid.y;
+    double* part = (double*)dh_ws(h, DH_WS_PARTIAL, sizeof(double) * (2 * nblk + 2));
+    if (!part) return DH_ERR_ALLOC;
+    lra_error_f64_kernel<<<grid, 256, 0, h->stream>>>(n, d, l, X, ldx, nrm, y, gamma, Q, ldq, W,
+                                                      ldw, part);
+    DH_CHECK_LAUNCH();
+    reduce_pairs_kernel<<<1, 256, 0, h->stream>>>(nblk, part, part + 2 * nblk);
+    DH_CHECK_LAUNCH();
+    DH_CHECK_HIP(hipMemcpyAsync(err2, part + 2 * nblk, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                                h->stream));
+    DH_CHECK_HIP(hipStreamSynchronize(h->stream));
+    return 0;
+}

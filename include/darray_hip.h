@@ -1,0 +1,170 @@
+/*
+ * darray_hip.h -- C ABI of the MI355X (gfx950) RBF low-rank-approximation library.
+ *
+ * Plain C types only: pointers, sizes, scalars.  Every matrix is column-major
+ * with an explicit leading dimension, as DArray's LMatrix / DMatrix local
+ * block (darray.h:257-421, 424-453).  Point sets are d x n column-major, one
+ * point per column (DArray's training matrix X, testsvm.cpp:76).
+ *
+ * Memory: unless a function says otherwise, matrix arguments are DEVICE
+ * pointers and the call is asynchronous on the handle's stream.  The handle
+ * owns its device workspaces (allocated once, grown on demand).  Functions
+ * that return a LAPACK-style info synchronise the stream before returning.
+ *
+ * Status codes: 0 = success; > 0 = LAPACK-style info (order of the failing
+ * minor of a Cholesky, CholeskyQR breakdown); -1 .. -99 = -(index of the bad
+ * argument, 1-based, the handle counting as argument 1); <= -1000: see
+ * DH_ERR_*.  dh_last_error_string() describes the last failure on the
+ * calling thread.
+ *
+ * Threading: one handle per host thread; independent handles may be used
+ * concurrently.
+ */
+#ifndef DARRAY_HIP_H
+#define DARRAY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DH_ERR_HIP (-1000)         /* a HIP runtime call failed */
+#define DH_ERR_ALLOC (-1001)       /* device allocation failed */
+#define DH_ERR_UNSUPPORTED (-1002) /* shape outside what the kernels handle */
+
+typedef struct dh_handle_s* dh_handle_t;
+
+/* ---------------------------------------------------------------- handle */
+
+/* Create a handle on `device`.  `stream` is a hipStream_t to run on, or NULL
+ * for a stream owned by the handle.  Replaces cuda_init (dkernel.cu:28-34),
+ * which created global cuBLAS/cuSOLVER handles on one stream. */
+int dh_create(int device, void* stream, dh_handle_t* out);
+/* Destroy the handle and its workspaces (cuda_finalize, dkernel.cu:38-39). */
+int dh_destroy(dh_handle_t h);
+/* Block until all work queued on the handle's stream has finished. */
+int dh_synchronize(dh_handle_t h);
+/* Message for the last non-zero status returned on this thread. */
+const char* dh_last_error_string(void);
+/* Bytes of device workspace currently held by the handle. */
+size_t dh_workspace_bytes(dh_handle_t h);
+
+/* ------------------------------------------------------------- RBF kernel */
+
+/* Squared norms nrm[i] = sum_f X[f + i*ldx]^2 (vecnorm, dkernel.cu:102-114). */
+int dh_sqnorms_f64(dh_handle_t h, int n, int d, const double* X, int ldx, double* nrm);
+
+/* Materialised RBF tile, K[i + j*ldk] = yi[i]*yj[j]*exp(-gamma*max(0, |xi|^2+|xj|^2-2 xi.xj)),
+ * Xi d x m, Xj d x n (point per column); yi / yj may be NULL (all ones).
+ * Device counterpart of rbf_kernel (dkernel.h:6, dkernel.cu:132-190): the
+ * distance GEMM runs on MFMA and the exp in the GEMM epilogue, so the m x n
+ * inner-product matrix never reaches memory. */
+int dh_rbf_tile_f64(dh_handle_t h, int m, int n, int d, const double* Xi, int ldxi,
+                    const double* Xj, int ldxj, const double* yi, const double* yj,
+                    double gamma, double* K, int ldk);
+int dh_rbf_tile_f32(dh_handle_t h, int m, int n, int d, const float* Xi, int ldxi,
+                    const float* Xj, int ldxj, const float* yi, const float* yj, float gamma,
+                    float* K, int ldk);
+
+/* Fused Y[m x l] = K(Xi,Xj) * O[n x l] + beta*Y in FP64 (MFMA f64 for the distance
+ * and the projection, FP64 exp).  K is never stored.  Replaces the rbf_kernel +
+ * replicate + tc_gemm sequence of DArray::LRA (dkernel.h:33-42). */
+int dh_rbf_project_f64(dh_handle_t h, int m, int n, int d, int l, const double* Xi, int ldxi,
+                       const double* Xj, int ldxj, const double* yi, const double* yj,
+                       double gamma, const double* O, int ldo, double beta, double* Y, int ldy);
+
+/* ------------------------------------------------------------- dense BLAS */
+
+/* C = alpha*op(A)*op(B) + beta*C, op in {'N','T'}: tc_gemm / tc_sgemm
+ * (dkernel.h:7-8, dkernel.cu:194-275) and multiply_matrix (cblas_like.h:39-60).
+ * Unlike tc_gemm, beta is honoured. */
+int dh_gemm_f64(dh_handle_t h, char transa, char transb, int m, int n, int k, double alpha,
+                const double* A, int lda, const double* B, int ldb, double beta, double* C,
+                int ldc);
+int dh_gemm_f32(dh_handle_t h, char transa, char transb, int m, int n, int k, float alpha,
+                const float* A, int lda, const float* B, int ldb, float beta, float* C,
+                int ldc);
+
+/* Upper SYRK C = alpha*A^T A + beta*C on the upper triangle (incl. diagonal) of the
+ * n x n C, A k x n; the strict lower triangle is not written
+ * (symmetric_update -> ssyrk_("U","T"), cblas_like.h:61-69). */
+int dh_syrk_ut_f64(dh_handle_t h, int n, int k, double alpha, const double* A, int lda,
+                   double beta, double* C, int ldc);
+int dh_syrk_ut_f32(dh_handle_t h, int n, int k, float alpha, const float* A, int lda,
+                   float beta, float* C, int ldc);
+
+/* Triangular solve with the upper triangle of A: side 'L' (B m x n, A m x m) or
+ * 'R' (A n x n), trans 'N' or 'T', diag 'N' or 'U':
+ *   L,U,T: B := alpha*U^{-T} B   (triangular_solve, blas_like.h:95-128)
+ *   L,U,N: B := alpha*U^{-1} B   (no reference counterpart; blas_like.h:124-126)
+ *   R,U,N: B := alpha*B U^{-1}   (no reference counterpart; CholeskyQR)
+ * uplo must be 'U' (the reference implements only the upper case). */
+int dh_trsm_f64(dh_handle_t h, char side, char uplo, char transa, char diag, int m, int n,
+                double alpha, const double* A, int lda, double* B, int ldb);
+int dh_trsm_f32(dh_handle_t h, char side, char uplo, char transa, char diag, int m, int n,
+                float alpha, const float* A, int lda, float* B, int ldb);
+
+/* Right-looking blocked Cholesky A = U^T U of the n x n upper triangle, block bs
+ * (cholesky_factorize, lapack_like.h:9-56: potrf(A11) -> trsm(L,U,T) -> syrk).
+ * The strict lower triangle is never written.  Returns 0 or info > 0, the
+ * global 1-based order of the first non-positive pivot; A is then partially
+ * factored (lapack_like.h:25-28).  Synchronises.  uplo must be 'U'. */
+int dh_potrf_f64(dh_handle_t h, char uplo, int n, double* A, int lda, int bs);
+int dh_potrf_f32(dh_handle_t h, char uplo, int n, float* A, int lda, int bs);
+
+/* ------------------------------------------------------- range finder */
+
+/* CholeskyQR2 in FP64: Y (m x n, m >= n) is overwritten by Q with Q^T Q = I, and
+ * R (n x n upper, positive diagonal; may be NULL) receives Y = Q R.  Returns
+ * 0, or > 0 if a Gram matrix is not numerically positive definite (the potrf
+ * info; 1000000 + info in the second sweep).  Replaces tall_skinny_qr_factorize
+ * (lapack_like.h:281-379) on the path.  Synchronises. */
+int dh_cholqr2_f64(dh_handle_t h, int m, int n, double* Y, int ldy, double* R, int ldr);
+
+/* Precision of the K-passes of dh_lra. */
+#define DH_PREC_F64 0  /* FP64 MFMA distance + projection, FP64 exp */
+#define DH_PREC_BF16 1 /* BF16 MFMA distance + projection, FP32 accumulate and exp */
+
+/* Randomised range finder on the RBF kernel of one point set (DArray::LRA,
+ * dkernel.h:12-46, and the orthogonalisation of testsvm.cpp:102-108, fused):
+ *   Y = K*Omega; Q = CholQR2(Y); q times { Y = K*Q; Q = CholQR2(Y) }; W = K*Q
+ * so B = Q^T K = W^T.  X d x n, y labels (n, may be NULL), Omega n x l.
+ * Outputs Q (n x l, FP64) and W (n x l, FP64).  `prec` selects the K-pass
+ * arithmetic; CholeskyQR2 is FP64 in both.  Synchronises. */
+int dh_lra(dh_handle_t h, int prec, int n, int d, int l, int q, const double* X, int ldx,
+           const double* y, double gamma, const double* Omega, int ldo, double* Q, int ldq,
+           double* W, int ldw);
+
+/* FP64 error pass: err2[0] = ||K - Q W^T||_F^2, err2[1] = ||K||_F^2, where err2 is a
+ * HOST pointer to 2 doubles.  K is regenerated on the fly from X.  This is the
+ * quantity Knorm (dkernel.h:49-76) means to print.  Synchronises. */
+int dh_lra_error(dh_handle_t h, int n, int d, int l, const double* X, int ldx, const double* y,
+                 double gamma, const double* Q, int ldq, const double* W, int ldw,
+                 double* err2);
+
+/* ---------------------------------------------------- host-side inputs */
+/* Seeded host generators (HOST pointers; no GPU use). */
+
+/* Raw 32-bit outputs of std::mt19937(seed). */
+int dh_mt19937_raw(uint32_t seed, int count, uint32_t* out);
+/* DMatrix::set_uniform with a fixed seed (darray.h:513-522): std::mt19937(seed) +
+ * uniform_real_distribution<double>(a,b), filled row-outer / column-inner into the
+ * column-major m x n A. */
+int dh_fill_uniform(uint32_t seed, double a, double b, int m, int n, double* A, int lda);
+/* DMatrix::set_normal_seed (darray.h:535-544): std::mt19937(seed) +
+ * normal_distribution<double>(mean, stddev), same fill order. */
+int dh_fill_normal(uint32_t seed, double mean, double stddev, int m, int n, double* A, int lda);
+/* Gaussian mixture (d x n, point per column): std::mt19937_64(seed); centres
+ * uniform in [-1,1]^d (centre-major), point -> centre assignment uniform, then
+ * point = centre + sigma*N(0,I) (point-major).  labels (n, may be NULL) receive
+ * +1 if the nearest centre has an even index, else -1; centres (d x ncentres,
+ * may be NULL) receive the centres. */
+int dh_fill_mixture(uint64_t seed, int d, int n, int ncentres, double sigma, double* X, int ldx,
+                    double* labels, double* centres);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,140 @@
+/*
+ * oracle.h -- CPU restatement (FP64, plain C) of DArray's RBF low-rank
+ * approximation path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * The product (darray_optimizer_amd/, include/) never links or calls it.
+ *
+ * Conventions (as in DArray, darray.h:257-421): every matrix is column-major
+ * with an explicit leading dimension.  Point sets X are stored the way
+ * DArray stores its training matrix, a d x n DMatrix (testsvm.cpp:76):
+ * column i holds the d features of point i, so X[f + i*ldx] is feature f of
+ * point i ("point-major").
+ *
+ * Every function cites the reference file:line whose behaviour it restates.
+ * Where the reference is buggy the restatement follows the evident intent
+ * and the deviation is stated next to the function.
+ */
+#ifndef DARRAY_ORACLE_H
+#define DARRAY_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- RBF tile (dkernel.cu:102-129, 132-190; tests/test_barrier.cpp:438-452) ---- */
+
+/* nrm[i] = sum_f X[f + i*ldx]^2 (vecnorm, dkernel.cu:102-114), summed in feature order. */
+void orc_sqnorms(int n, int d, const double* X, int ldx, double* nrm);
+
+/* K[i + j*ldk] = yi[i]*yj[j]*exp(-gamma*(|xi|^2 + |xj|^2 - 2 xi.xj))  (rbf, dkernel.cu:116-129).
+ * xi.xj is the (N,T) distance GEMM of dkernel.cu:160-169 / test_barrier.cpp:438.
+ * yi / yj may be NULL (labels == 1).  clamp != 0 clamps the squared distance at 0
+ * (the reference does not clamp). */
+void orc_rbf_tile(int m, int n, int d, const double* Xi, int ldxi, const double* Xj, int ldxj,
+                  const double* yi, const double* yj, double gamma, int clamp,
+                  double* K, int ldk);
+
+/* Y[m x l] = K(Xi, Xj) * O[n x l] + beta*Y, generated and consumed in row/column
+ * chunks of `chunk` points, as DArray::LRA does with its NN x NN blocks
+ * (dkernel.h:12-46).  Deviations from dkernel.h, both bug fixes: every chunk
+ * is processed (dkernel.h:43 returns after the first), and beta is honoured
+ * on the first column chunk (tc_gemm ignores beta, dkernel.cu:201). */
+void orc_rbf_project(int m, int n, int d, int l, const double* Xi, int ldxi,
+                     const double* Xj, int ldxj, const double* yi, const double* yj,
+                     double gamma, const double* O, int ldo, double beta, double* Y, int ldy,
+                     int chunk);
+
+/* ---- dense kernels (cblas_like.h / blas_like.h / lapack_like.h) ---- */
+
+/* C = alpha*op(A)*op(B) + beta*C, op in {'N','T'} (multiply_matrix, cblas_like.h:39-60). */
+void orc_gemm(char ta, char tb, int m, int n, int k, double alpha, const double* A, int lda,
+              const double* B, int ldb, double beta, double* C, int ldc);
+
+/* C = alpha*A^T*B + beta*C computed in k-blocks of bs rows with beta forced to 1
+ * after the first block (matrix_multiply "T","N", blas_like.h:62-93). */
+void orc_matrix_multiply_tn(int m, int n, int k, double alpha, const double* A, int lda,
+                            const double* B, int ldb, double beta, double* C, int ldc, int bs);
+
+/* Upper SYRK: C[upper incl. diag] = alpha*A^T*A + beta*C, A is k x n
+ * (symmetric_update -> ssyrk_("U","T"), cblas_like.h:61-69).  Strict lower of C untouched. */
+void orc_syrk_ut(int n, int k, double alpha, const double* A, int lda, double beta,
+                 double* C, int ldc);
+
+/* Unblocked upper Cholesky A = U^T U in place (LAPACK potrf "U" contract used by
+ * spotrf_, lapack_like.h:24).  Returns 0 or the 1-based order of the failing minor. */
+int orc_potf2_upper(int n, double* A, int lda);
+
+/* Right-looking blocked upper Cholesky (cholesky_factorize, lapack_like.h:9-56):
+ * per block: potrf(A11) -> trsm("L","U","T","N", A11, A12) -> syrk("U","T",-1, A12, A22).
+ * The strictly lower triangle is never referenced or written.  On failure returns
+ * the global 1-based failing order and stops, leaving A partially factored
+ * (lapack_like.h:25-28). */
+int orc_potrf_upper(int n, double* A, int lda, int bs);
+
+/* Recursive-halving upper Cholesky (cholesky_factorize_recursive, lapack_like.h:58-90):
+ * base case n <= bs -> potrf; else factor A11, trsm(L,U,T) on A12, A22 -= A12^T A12
+ * via matrix_multiply("T","N"), recurse on A22. */
+int orc_potrf_upper_recursive(int n, double* A, int lda, int bs);
+
+/* B := alpha * U^{-T} B, U = upper triangle of A (m x m), B m x n
+ * (triangular_solve "L","U","T", blas_like.h:95-128: recursion m/2, base m <= bs,
+ * update by matrix_multiply("T","N",-1,1)).  Deviation: alpha is applied
+ * (the reference passes 1 to strsm_, blas_like.h:98,110).  diag 'U' = unit diagonal. */
+void orc_trsm_lut(char diag, int m, int n, double alpha, const double* A, int lda,
+                  double* B, int ldb, int bs);
+
+/* B := alpha * U^{-1} B (side L, upper, no-transpose).  No reference counterpart
+ * (blas_like.h:124-126 asserts NOT_IMPLEMENTED); needed for the U x = y half of the
+ * core solve.  Recursion mirrors orc_trsm_lut with the halves processed bottom-up. */
+void orc_trsm_lun(char diag, int m, int n, double alpha, const double* A, int lda,
+                  double* B, int ldb, int bs);
+
+/* B := alpha * B * U^{-1} (side R, upper, no-transpose), B m x n, U n x n.
+ * No reference counterpart; used by CholeskyQR (Q = Y R^{-1}). */
+void orc_trsm_run(char diag, int m, int n, double alpha, const double* A, int lda,
+                  double* B, int ldb);
+
+/* ---- orthogonalisation ---- */
+
+/* Householder QR with explicit Q, the one-rank case of tall_skinny_qr_factorize
+ * (lapack_like.h:281-379): geqrf of the local block, QR of the (single) stacked R
+ * -- an upper-triangular matrix, whose reflectors are identities -- and ormqr
+ * applied to [I;0].  A (m x n, m >= n) is overwritten by Q, R (n x n) written to R.
+ * Reflectors follow LAPACK dlarfg: R[j][j] = -sign(a_jj)*norm. */
+void orc_householder_qr(int m, int n, double* A, int lda, double* R, int ldr);
+
+/* CholeskyQR2: G = Y^T Y, R1 = chol(G), Q1 = Y R1^{-1}; again on Q1 -> R2, Q = Q1 R2^{-1};
+ * R = R2 R1.  Y (m x n) overwritten by Q.  Returns 0 or the potrf info (> 0) of the
+ * sweep that broke down (1000000 + info for the second sweep).  No reference
+ * counterpart; it replaces tall_skinny_qr_factorize on the path. */
+int orc_cholqr2(int m, int n, double* Y, int ldy, double* R, int ldr);
+
+/* ---- the LRA pipeline (testsvm.cpp:86-177 order, plus power iterations) ---- */
+
+/* Randomised range finder on the RBF kernel of one point set X (d x n):
+ *   Y = K*Omega ; Q = orth(Y) ; repeat q times { Y = K*Q ; Q = orth(Y) } ; W = K*Q
+ * so that B = Q^T K = W^T (K symmetric).  orth = 0: Householder (DArray's TSQR),
+ * orth = 1: CholeskyQR2.  Q (n x l, ldq) and W (n x l, ldw) are outputs;
+ * Omega is read only.  Returns 0 or the orthogonalisation info. */
+int orc_lra(int n, int d, int l, int q, const double* X, int ldx, const double* y,
+            double gamma, const double* Omega, int ldo, int orth, int chunk,
+            double* Q, int ldq, double* W, int ldw);
+
+/* ||K - Q W^T||_F and ||K||_F accumulated in double over row chunks
+ * (Frobenius norm of darray.h:607-617; the quantity Knorm, dkernel.h:49-76,
+ * intends to print).  Results in err2[0] = sum (K-QW^T)^2, err2[1] = sum K^2. */
+void orc_lra_error(int n, int d, int l, const double* X, int ldx, const double* y, double gamma,
+                   const double* Q, int ldq, const double* W, int ldw, int chunk, double* err2);
+
+/* Threads used by the OpenMP-parallel loops (1 = the sequential-BLAS link DArray uses). */
+void orc_set_threads(int nthreads);
+int orc_get_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
