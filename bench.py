@@ -51,14 +51,15 @@ def cpu_baseline(X, Om, gamma, rows, threads):
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps-fp64", type=int, default=1)
     ap.add_argument("--n", type=int, default=65536)
     ap.add_argument("--d", type=int, default=128)
     ap.add_argument("--k", type=int, default=256)
     ap.add_argument("--p", type=int, default=16)
     ap.add_argument("--q", type=int, default=2)
-    ap.add_argument("--cpu-rows", type=int, default=1024)
+    ap.add_argument("--cpu-rows", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     a = ap.parse_args(argv)
@@ -96,40 +97,57 @@ def main(argv=None) -> int:
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps
 
-    s64 = timed(dh.DH_PREC_F64, a.steps, a.warmup)
-    res64 = {"s_per_solve": s64, "gflops": (f_mfma + f_orth) / s64 / 1e9}
-    if not a.no_error:
-        e2, k2 = h.lra_error(Xd, gamma, Q, W)
-        res64["rel_err"] = float(np.sqrt(max(e2, 0.0) / k2))
+    res = {}
+    for name, prec, steps, warm in (("fp64", dh.DH_PREC_F64, a.steps_fp64, min(a.warmup, 1)),
+                                    ("mixed", dh.DH_PREC_BF16, a.steps, a.warmup)):
+        s = timed(prec, steps, warm)
+        r = {"s_per_solve": s, "gflops": (f_mfma + f_orth) / s / 1e9, "steps": steps}
+        if not a.no_error:
+            e2, k2 = h.lra_error(Xd, gamma, Q, W)
+            r["rel_err"] = float(np.sqrt(max(e2, 0.0) / k2))
+        res[name] = r
+    mixed, fp64 = res["mixed"], res["fp64"]
 
-    # Dominant kernel: one fused FP64 K*Omega pass, timed with events on the
-    # stream it runs on.
+    # Dominant kernel of each path: one fused K*Omega pass, timed with events
+    # on the stream it runs on (the handle runs on torch's current stream).
     Yd = torch.empty((l, n), dtype=torch.float64, device="cuda")
-    h.rbf_project_f64(Xd, Xd, Omd, gamma, Yd)
-    reps = 3
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(reps):
-        h.rbf_project_f64(Xd, Xd, Omd, gamma, Yd)
-    e1.record(stream)
-    e1.synchronize()
-    t_pass = e0.elapsed_time(e1) / 1e3 / reps
     pass_flops = 2.0 * n * n * d + 2.0 * n * n * l
-    achieved = pass_flops / t_pass
-    roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64_MFMA / 1e12,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA, "traffic": None,
-                "kernel": "rbf_project_f64_kernel<32,17>", "kernel_ms": t_pass * 1e3}
+
+    def pass_time(fn, reps):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / reps
+
+    t_bf16 = pass_time(lambda: h.rbf_project_bf16(Xd, Xd, Omd, gamma, Yd), 10)
+    t_f64 = pass_time(lambda: h.rbf_project_f64(Xd, Xd, Omd, gamma, Yd), 2)
+    roofline = {"bound": "mfma", "achieved": pass_flops / t_bf16 / 1e12,
+                "peak": PEAK_BF16_MFMA / 1e12, "unit": "TFLOP/s",
+                "frac": pass_flops / t_bf16 / PEAK_BF16_MFMA, "traffic": None,
+                "kernel": "rbf_project_bf16_kernel<4,17,8>", "kernel_ms": t_bf16 * 1e3}
+    fp64["roofline"] = {"bound": "mfma", "achieved": pass_flops / t_f64 / 1e12,
+                        "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s",
+                        "frac": pass_flops / t_f64 / PEAK_FP64_MFMA,
+                        "kernel": "rbf_project_f64_kernel<32,17>", "kernel_ms": t_f64 * 1e3}
 
     line = {
         "metric": "RBF-LRA GFLOP/s & s/solve, mixed vs FP64 on 1 MI355X at matched ||K-QB||/||K||",
-        "value": res64["gflops"], "unit": "GFLOP/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": s64 * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "value": mixed["gflops"], "unit": "GFLOP/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": mixed["s_per_solve"] * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
         "config": {"workload": "rbf_lra_gaussian_mixture", "n": n, "d": d, "k": a.k, "p": a.p,
                    "q": q, "l": l, "gamma": gamma, "centres": 16, "sigma": 0.25,
-                   "precision": "fp64"},
-        "fp64": res64, "roofline": roofline,
+                   "precision": "bf16 K-passes (fp32 accumulate), fp64 CholeskyQR2"},
+        "mixed": mixed, "fp64": fp64,
+        "speedup_vs_fp64": fp64["s_per_solve"] / mixed["s_per_solve"],
+        "roofline": roofline,
     }
+    if "rel_err" in mixed:
+        line["err_ratio_vs_fp64"] = mixed["rel_err"] / fp64["rel_err"]
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(X, Om, gamma, a.cpu_rows, threads)

@@ -26,8 +26,11 @@ def _ptr(t):
 
 def _ld(t):
     """Leading dimension of a column-major matrix held as a (cols, rows) tensor."""
-    assert t.dim() == 2 and t.stride(1) == 1, "expected a (cols, rows) tensor, rows contiguous"
-    return max(1, t.stride(0))
+    assert t.dim() == 2, "expected a 2-D (cols, rows) tensor"
+    if t.numel() == 0:
+        return max(1, t.shape[1])
+    assert t.stride(1) == 1 or t.shape[1] == 1, "rows must be contiguous"
+    return max(1, t.stride(0), t.shape[1])
 
 
 def to_dev(a: np.ndarray, dtype=None, device="cuda"):
@@ -99,6 +102,15 @@ class Handle:
         l = O.shape[0]
         call("dh_rbf_project_f64", self.h, m, n, d, l, _ptr(Xi), _ld(Xi), _ptr(Xj), _ld(Xj),
              _ptr(yi), _ptr(yj), float(gamma), _ptr(O), _ld(O), float(beta), _ptr(Y), _ld(Y))
+        return Y
+
+    def rbf_project_bf16(self, Xi, Xj, O, gamma, Y, yi=None, yj=None):
+        """Y (m x l, FP64) = K(Xi, Xj) O through the BF16 MFMA path (FP64 in/out)."""
+        m, d = Xi.shape
+        n = Xj.shape[0]
+        l = O.shape[0]
+        call("dh_rbf_project_bf16", self.h, m, n, d, l, _ptr(Xi), _ld(Xi), _ptr(Xj), _ld(Xj),
+             _ptr(yi), _ptr(yj), float(gamma), _ptr(O), _ld(O), _ptr(Y), _ld(Y))
         return Y
 
     # -------------------------------------------------------------- dense

@@ -25,6 +25,8 @@ SIGNATURES: dict[str, tuple] = {
     "dh_rbf_tile_f32": (_i, [_p, _i, _i, _i, _p, _i, _p, _i, _p, _p, _f, _p, _i]),
     "dh_rbf_project_f64": (_i, [_p, _i, _i, _i, _i, _p, _i, _p, _i, _p, _p, _d, _p, _i, _d, _p,
                                 _i]),
+    "dh_rbf_project_bf16": (_i, [_p, _i, _i, _i, _i, _p, _i, _p, _i, _p, _p, _d, _p, _i, _p,
+                                 _i]),
     "dh_gemm_f64": (_i, [_p, _c, _c, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p, _i]),
     "dh_gemm_f32": (_i, [_p, _c, _c, _i, _i, _i, _f, _p, _i, _p, _i, _f, _p, _i]),
     "dh_syrk_ut_f64": (_i, [_p, _i, _i, _d, _p, _i, _d, _p, _i]),

@@ -45,7 +45,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(int m, int n, int k, T alpha,
         for (int b = 0; b < FN; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                int i = m0 + tile_row<BM, BN>(a, r);
+                int i = m0 + tile_row<T, BM, BN>(a, r);
                 int j = n0 + tile_col<BM, BN>(b);
                 if (i < m && j < n && (!uplo_upper || i <= j)) {
                     T v = acc[a][b][r];

@@ -25,40 +25,140 @@ __global__ __launch_bounds__(256) void triu_copy_kernel(int n, const double* __r
     dst[idx] = (i <= j) ? src[i + (size_t)j * lds] : 0.0;
 }
 
-static int cholqr_sweep(dh_handle_t h, int m, int n, double* Y, int ldy, double* G, int* info) {
-    // G = upper(Y^T Y) (SYRK, split over m), G = U^T U, Y := Y U^{-1}.
+// Device scalars of the CholeskyQR checks live after the potrf status word.
+constexpr size_t kCqDmaxOffset = 128;  // bytes into DH_WS_INFO: double max diag(G)
+
+// dmax = max_k G[k][k]; with shift != 0 also G += s*I where
+// s = 11 (m n + n (n+1)) u trace(G) -- the shift of shifted CholeskyQR3
+// (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa, SIAM J. Sci. Comput.
+// 2020) with ||Y||_2^2 bounded by ||Y||_F^2 = trace(G).
+__global__ __launch_bounds__(256) void gram_diag_kernel(int m, int n, double* __restrict__ G,
+                                                         int ldg, double* __restrict__ dmax,
+                                                         int shift) {
+    __shared__ double smax[256], ssum[256];
+    double mx = 0.0, sm = 0.0;
+    for (int k = threadIdx.x; k < n; k += 256) {
+        double v = G[k + (size_t)k * ldg];
+        mx = v > mx ? v : mx;
+        sm += v;
+    }
+    smax[threadIdx.x] = mx;
+    ssum[threadIdx.x] = sm;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + st]);
+            ssum[threadIdx.x] += ssum[threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (shift) {
+        double s = 11.0 * ((double)m * n + (double)n * (n + 1)) * 1.1102230246251565e-16 * ssum[0];
+        for (int k = threadIdx.x; k < n; k += 256) G[k + (size_t)k * ldg] += s;
+    }
+    if (threadIdx.x == 0) *dmax = smax[0];
+}
+
+// Numerical breakdown test after a successful potrf: a pivot with
+// U_kk^2 <= n u max_j G_jj means Y is rank deficient to working precision
+// (kappa(Y) beyond ~u^{-1/2}); record the first such k (1-based) in *info.
+__global__ __launch_bounds__(256) void cholqr_check_kernel(int n, const double* __restrict__ U,
+                                                            int ldu,
+                                                            const double* __restrict__ dmax,
+                                                            int* __restrict__ info) {
+    double tol = (double)n * 2.220446049250313e-16 * (*dmax);
+    for (int k = threadIdx.x; k < n; k += 256) {
+        double u = U[k + (size_t)k * ldu];
+        if (!(u * u > tol)) atomicMin(info, k + 1);
+    }
+}
+
+__global__ void reset_check_kernel(int* info) { *info = 0x7fffffff; }
+
+// One CholeskyQR sweep on Y (m x n): G = upper(Y^T Y) (SYRK split over m),
+// optional shift, G = U^T U, breakdown check, Y := Y U^{-1}.  On return
+// *info is 0 or the 1-based index of the first failing pivot; U stays in G.
+static int cholqr_sweep(dh_handle_t h, int m, int n, double* Y, int ldy, double* G, bool shift,
+                        int* info) {
+    char* ws = (char*)dh_ws(h, DH_WS_INFO, 256);
+    if (!ws) return DH_ERR_ALLOC;
+    double* dmax = (double*)(ws + kCqDmaxOffset);
+    int* chk = (int*)(ws + kCqDmaxOffset + 64);
     DH_CHECK_STATUS(dh_gemm_impl<double>(h, true, false, n, n, m, 1.0, Y, ldy, Y, ldy, 0.0, G, n, 1));
+    gram_diag_kernel<<<1, 256, 0, h->stream>>>(m, n, G, n, dmax, shift ? 1 : 0);
+    DH_CHECK_LAUNCH();
     DH_CHECK_STATUS(dh_potrf_impl<double>(h, n, G, n, 64, info));
     if (*info) return 0;
+    reset_check_kernel<<<1, 1, 0, h->stream>>>(chk);
+    DH_CHECK_LAUNCH();
+    cholqr_check_kernel<<<1, 256, 0, h->stream>>>(n, G, n, dmax, chk);
+    DH_CHECK_LAUNCH();
+    int hchk = 0;
+    DH_CHECK_HIP(hipMemcpyAsync(&hchk, chk, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    DH_CHECK_HIP(hipStreamSynchronize(h->stream));
+    if (hchk != 0x7fffffff) {
+        *info = hchk;
+        return 0;
+    }
     return dh_trsm_impl<double>(h, 'R', 'N', false, m, n, 1.0, G, n, Y, ldy);
 }
 
+// CholeskyQR2; if the first Gram matrix is numerically singular, shifted
+// CholeskyQR3 (shifted sweep + two plain sweeps).  R (optional) = product of
+// the sweep factors, last one leftmost.
 int dh_cholqr2_impl(dh_handle_t h, int m, int n, double* Y, int ldy, double* R, int ldr) {
     if (n == 0) return 0;
     size_t nn = (size_t)n * n;
     double* G = (double*)dh_ws(h, DH_WS_GRAM, sizeof(double) * nn * 2);
     if (!G) return DH_ERR_ALLOC;
-    double* R1 = G + nn;
-    int info = 0;
-    DH_CHECK_STATUS(cholqr_sweep(h, m, n, Y, ldy, G, &info));
-    if (info) return dh_fail(info, "CholeskyQR2: Gram matrix of sweep 1 not positive definite (minor %d)", info);
+    double* Racc = G + nn;
+    double* Rtmp = nullptr;
     if (R) {
-        triu_copy_kernel<<<dh_cdiv((long)nn, 256), 256, 0, h->stream>>>(n, G, n, R1);
-        DH_CHECK_LAUNCH();
+        Rtmp = (double*)dh_ws(h, DH_WS_RBUF, sizeof(double) * nn);
+        if (!Rtmp) return DH_ERR_ALLOC;
     }
-    DH_CHECK_STATUS(cholqr_sweep(h, m, n, Y, ldy, G, &info));
-    if (info)
-        return dh_fail(1000000 + info,
-                       "CholeskyQR2: Gram matrix of sweep 2 not positive definite (minor %d)", info);
-    if (R) {
-        // R = R2 * R1 (both upper): the lower triangle of G still holds Gram
-        // entries, so copy its upper part out first.
-        double* R2 = (double*)dh_ws(h, DH_WS_RBUF, sizeof(double) * nn);
-        if (!R2) return DH_ERR_ALLOC;
-        triu_copy_kernel<<<dh_cdiv((long)nn, 256), 256, 0, h->stream>>>(n, G, n, R2);
-        DH_CHECK_LAUNCH();
-        DH_CHECK_STATUS(
-            dh_gemm_impl<double>(h, false, false, n, n, n, 1.0, R2, n, R1, n, 0.0, R, ldr, 0));
+    int info = 0;
+    DH_CHECK_STATUS(cholqr_sweep(h, m, n, Y, ldy, G, false, &info));
+    int sweeps = 2;
+    if (info) {
+        // Y is untouched by a failed sweep: restart with the shifted Gram.
+        DH_CHECK_STATUS(cholqr_sweep(h, m, n, Y, ldy, G, true, &info));
+        if (info)
+            return dh_fail(info, "CholeskyQR: shifted Gram matrix not positive definite (pivot %d)",
+                           info);
+        sweeps = 3;
+    }
+    for (int sw = 1; sw <= sweeps; ++sw) {
+        if (sw > 1) {
+            DH_CHECK_STATUS(cholqr_sweep(h, m, n, Y, ldy, G, false, &info));
+            if (info)
+                return dh_fail(1000000 * (sw - 1) + info,
+                               "CholeskyQR: Gram matrix of sweep %d is numerically singular "
+                               "(pivot %d)", sw, info);
+        }
+        if (R) {
+            if (sw == 1) {
+                triu_copy_kernel<<<dh_cdiv((long)nn, 256), 256, 0, h->stream>>>(n, G, n, Racc);
+                DH_CHECK_LAUNCH();
+            } else {
+                triu_copy_kernel<<<dh_cdiv((long)nn, 256), 256, 0, h->stream>>>(n, G, n, Rtmp);
+                DH_CHECK_LAUNCH();
+                double* out = (sw == sweeps) ? R : Racc;
+                int ldo = (sw == sweeps) ? ldr : n;
+                if (sw == sweeps) {
+                    DH_CHECK_STATUS(dh_gemm_impl<double>(h, false, false, n, n, n, 1.0, Rtmp, n,
+                                                         Racc, n, 0.0, out, ldo, 0));
+                } else {
+                    // Racc := Rtmp * Racc via a temporary (GEMM output may not alias).
+                    double* t2 = (double*)dh_ws(h, DH_WS_PARTIAL, sizeof(double) * nn);
+                    if (!t2) return DH_ERR_ALLOC;
+                    DH_CHECK_STATUS(dh_gemm_impl<double>(h, false, false, n, n, n, 1.0, Rtmp, n,
+                                                         Racc, n, 0.0, t2, n, 0));
+                    DH_CHECK_HIP(hipMemcpyAsync(Racc, t2, sizeof(double) * nn,
+                                                hipMemcpyDeviceToDevice, h->stream));
+                }
+            }
+        }
     }
     return 0;
 }

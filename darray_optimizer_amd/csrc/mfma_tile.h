@@ -1,8 +1,9 @@
 // MFMA tile accumulation shared by the GEMM, RBF-tile and error-pass kernels.
 //
-// v_mfma_f64_16x16x4_f64 and v_mfma_f32_16x16x4_f32 share one operand layout:
-// for a 16x16x4 block, lane L holds A[L%16][L/16], B[L/16][L%16] and four
-// accumulators D[4*(L/16)+r][L%16], r = 0..3.
+// v_mfma_f64_16x16x4_f64 and v_mfma_f32_16x16x4_f32 share the input layout:
+// for a 16x16x4 block, lane L holds A[L%16][L/16] and B[L/16][L%16].  They
+// differ in where the four accumulators of lane L sit: f32 holds
+// D[4*(L/16)+r][L%16], f64 holds D[L/16+4*r][L%16], r = 0..3 (Mfma<T>::row).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -18,6 +19,8 @@ struct Mfma<double> {
     static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    // Row of accumulator r of a lane in lane group g = L/16.
+    static __device__ __forceinline__ int row(int g, int r) { return g + 4 * r; }
 };
 template <>
 struct Mfma<float> {
@@ -25,6 +28,7 @@ struct Mfma<float> {
     static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
 };
 
 constexpr int kTileBK = 16;
@@ -94,10 +98,10 @@ __device__ __forceinline__ void tile_mma(typename Mfma<T>::acc_t (&acc)[BM / 32]
 }
 
 // Row / column of accumulator element (a, b, r) for this lane within the tile.
-template <int BM, int BN>
+template <typename T, int BM, int BN>
 __device__ __forceinline__ int tile_row(int a, int r) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    return (wave & 1) * (BM / 2) + a * 16 + 4 * (lane >> 4) + r;
+    return (wave & 1) * (BM / 2) + a * 16 + Mfma<T>::row(lane >> 4, r);
 }
 template <int BM, int BN>
 __device__ __forceinline__ int tile_col(int b) {

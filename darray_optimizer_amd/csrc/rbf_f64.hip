@@ -80,7 +80,7 @@ __global__ __launch_bounds__(256) void rbf_tile_kernel(int m, int n, int d,
         for (int b = 0; b < FN; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                int i = m0 + tile_row<BM, BN>(a, r);
+                int i = m0 + tile_row<T, BM, BN>(a, r);
                 int j = n0 + tile_col<BM, BN>(b);
                 if (i < m && j < n) {
                     T d2 = ni[i] + nj[j] - T(2) * acc[a][b][r];
@@ -137,9 +137,9 @@ extern "C" int dh_rbf_tile_f32(dh_handle_t h, int m, int n, int d, const float* 
 //      A operand from LDS (Xj^T, f-major), B operand from registers (Xi);
 //   2. turns its 4 accumulators into K values in place (exp epilogue);
 //   3. uses them directly as the A operand of the projection MFMAs: lane L
-//      holds S^T[4*(L/16)+r][L%16] = K[i=L%16][j=4*(L/16)+r], i.e. MFMA r
-//      contracts over j = {r, 4+r, 8+r, 12+r}; Omega is staged in LDS in the
-//      matching order sO[r][cb][L] = Omega[j0+4*(L/16)+r][c0+16*cb+L%16].
+//      holds S^T[L/16+4r][L%16] = K[i=L%16][j=4r+L/16] (f64 accumulator
+//      layout), i.e. MFMA r contracts over j = 4r..4r+3; Omega is staged in
+//      LDS in the matching order sO[r][cb][L] = Omega[j0+4r+L/16][c0+16cb+L%16].
 // Labels are folded: yj scales Omega as it is staged, yi scales the output.
 // The projection accumulators (NCB blocks of 16 columns) stay in registers
 // for the whole j loop; Y is written once.
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(256) void rbf_project_f64_kernel(
                 v = O[j + (size_t)c * ldo];
                 if (yj) v *= yj[j];
             }
-            sO[jj & 3][cl >> 4][(jj >> 2) * 16 + (cl & 15)] = v;
+            sO[jj >> 2][cl >> 4][(jj & 3) * 16 + (cl & 15)] = v;
         }
         if (tid < JB) sN[tid] = (j0 + tid < n) ? nj[j0 + tid] : 0.0;
         __syncthreads();
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(256) void rbf_project_f64_kernel(
         double kv[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            double d2 = nrow + sN[4 * g + r] - 2.0 * s[r];
+            double d2 = nrow + sN[g + 4 * r] - 2.0 * s[r];
             d2 = d2 > 0.0 ? d2 : 0.0;
             kv[r] = exp(-gamma * d2);
         }
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(256) void rbf_project_f64_kernel(
     for (int cb = 0; cb < NCB; ++cb)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            int i = rbase + 4 * g + r, c = c0 + 16 * cb + li;
+            int i = rbase + g + 4 * r, c = c0 + 16 * cb + li;
             if (i < m && c < l) {
                 double v = acc[cb][r];
                 if (yi) v *= yi[i];
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(256) void lra_error_f64_kernel(
         for (int b = 0; b < FN; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                int i = m0 + tile_row<BM, BN>(a, r);
+                int i = m0 + tile_row<double, BM, BN>(a, r);
                 int j = n0 + tile_col<BM, BN>(b);
                 if (i < n && j < n) {
                     double d2 = nrm[i] + nrm[j] - 2.0 * S[a][b][r];

@@ -75,6 +75,17 @@ int dh_rbf_project_f64(dh_handle_t h, int m, int n, int d, int l, const double* 
                        const double* Xj, int ldxj, const double* yi, const double* yj,
                        double gamma, const double* O, int ldo, double beta, double* Y, int ldy);
 
+/* Mixed-precision fused Y[m x l] = K(Xi,Xj) * O (Y overwritten): points centred by
+ * the mean of Xj and rounded to BF16, FP32 squared norms of the rounded points,
+ * BF16 MFMA distance with FP32 accumulation, K = exp2(..) on v_exp_f32 rounded
+ * to BF16, O rounded to BF16 (labels folded in), BF16 MFMA projection with FP32
+ * accumulation, Y written in FP64.  d <= 256.  FP64 inputs; conversions run
+ * on the device.  Tensor-core counterpart of rbf_kernel + tc_gemm
+ * (dkernel.cu:160-166, 209-213; FP16 GemmEx precedent test_cuda.cu:111-132). */
+int dh_rbf_project_bf16(dh_handle_t h, int m, int n, int d, int l, const double* Xi, int ldxi,
+                        const double* Xj, int ldxj, const double* yi, const double* yj,
+                        double gamma, const double* O, int ldo, double* Y, int ldy);
+
 /* ------------------------------------------------------------- dense BLAS */
 
 /* C = alpha*op(A)*op(B) + beta*C, op in {'N','T'}: tc_gemm / tc_sgemm
@@ -117,10 +128,14 @@ int dh_potrf_f32(dh_handle_t h, char uplo, int n, float* A, int lda, int bs);
 /* ------------------------------------------------------- range finder */
 
 /* CholeskyQR2 in FP64: Y (m x n, m >= n) is overwritten by Q with Q^T Q = I, and
- * R (n x n upper, positive diagonal; may be NULL) receives Y = Q R.  Returns
- * 0, or > 0 if a Gram matrix is not numerically positive definite (the potrf
- * info; 1000000 + info in the second sweep).  Replaces tall_skinny_qr_factorize
- * (lapack_like.h:281-379) on the path.  Synchronises. */
+ * R (n x n upper, positive diagonal; may be NULL) receives Y = Q R.  A sweep
+ * breaks down when a Cholesky pivot satisfies U_kk^2 <= n*eps*max_j G_jj
+ * (Y rank deficient to working precision).  If the first sweep breaks down
+ * the call switches to shifted CholeskyQR3 (Fukaya et al., SIAM J. Sci.
+ * Comput. 2020).  Returns 0, or > 0 on breakdown: the 1-based pivot index,
+ * plus 1000000 for the second sweep and 2000000 for the third.  Replaces
+ * tall_skinny_qr_factorize (lapack_like.h:281-379) on the path.
+ * Synchronises. */
 int dh_cholqr2_f64(dh_handle_t h, int m, int n, double* Y, int ldy, double* R, int ldr);
 
 /* Precision of the K-passes of dh_lra. */

@@ -1,0 +1,108 @@
+"""Mixed-precision (BF16 MFMA) K-passes and LRA through the C ABI.
+
+Two bars per kernel: (1) against a numpy emulation of the same rounding
+scheme -- centred BF16 points, BF16 K, BF16 Omega, wide accumulation -- to
+1e-4 relative Frobenius (only FP32 accumulation order and exp2 rounding
+differ); (2) against the FP64 oracle to BF16 accuracy (5e-3)."""
+import numpy as np
+import pytest
+
+import darray_optimizer_amd as dh
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+def bf16(x):
+    """Round to the nearest bfloat16 (ties to even) through float32, as float64."""
+    f = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    u = f.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def emulate(Xi, Xj, O, gamma, yi=None, yj=None):
+    mu = Xj.mean(axis=0)
+    a = bf16(Xi - mu)
+    b = bf16(Xj - mu)
+    na = (a * a).sum(1)
+    nb = (b * b).sum(1)
+    d2 = np.maximum(na[:, None] + nb[None, :] - 2 * a @ b.T, 0)
+    K = bf16(np.exp(-gamma * d2))
+    Ow = O * (yj[:, None] if yj is not None else 1.0)
+    Y = K @ bf16(Ow)
+    if yi is not None:
+        Y = Y * yi[:, None]
+    return Y
+
+
+def dev_pts(X):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(X)).cuda()
+
+
+def dev_vec(v):
+    return None if v is None else dev_pts(v)
+
+
+SHAPES = [(1, 1, 1, 1), (64, 64, 64, 16), (100, 77, 13, 9), (300, 257, 130, 138),
+          (513, 1000, 128, 272), (200, 190, 200, 300), (96, 500, 256, 40)]
+
+
+@pytest.mark.parametrize("m,n,d,l", SHAPES)
+def test_rbf_project_bf16(dev, m, n, d, l):
+    import torch
+    rng = np.random.default_rng(3 * m + n + d + l)
+    Xi, Xj = rng.uniform(-1, 1, (m, d)), rng.uniform(-1, 1, (n, d))
+    yi, yj = rng.choice([-1.0, 1.0], m), rng.choice([-1.0, 1.0], n)
+    O = rng.standard_normal((n, l))
+    gamma = 1.0 / d
+    Y = torch.empty((l, m), dtype=torch.float64, device="cuda")
+    dev.rbf_project_bf16(dev_pts(Xi), dev_pts(Xj), dh.to_dev(O), gamma, Y, dev_vec(yi),
+                         dev_vec(yj))
+    Yg = dh.from_dev(Y)
+    assert rel(Yg, emulate(Xi, Xj, O, gamma, yi, yj)) < 1e-4
+    assert rel(Yg, orc.rbf_project(Xi, Xj, O, gamma, yi, yj)) < 5e-3
+
+
+def test_rbf_project_bf16_self_tile(dev):
+    # Xi is Xj (one conversion shared), mixture data at the C1 scale of gamma.
+    import torch
+    n, d, l = 2048, 128, 272
+    X = dh.fill_mixture(3, d, n, 16, 0.25)
+    O = np.asfortranarray(dh.fill_normal(4, n, l))
+    Xd = dev_pts(X)
+    Y = torch.empty((l, n), dtype=torch.float64, device="cuda")
+    dev.rbf_project_bf16(Xd, Xd, dh.to_dev(O), 0.5 / d, Y)
+    Yg = dh.from_dev(Y)
+    assert rel(Yg, emulate(X, X, O, 0.5 / d)) < 1e-4
+    assert rel(Yg, orc.rbf_project(X, X, O, 0.5 / d)) < 5e-3
+
+
+@pytest.mark.parametrize("labels", [False, True])
+def test_lra_mixed_error_close_to_fp64(dev, labels):
+    """Same seeded inputs through the FP64 and the mixed LRA: the mixed
+    approximation error stays within a small factor of the FP64 one."""
+    import torch
+    n, d, l, q = 4096, 64, 74, 1
+    X, y = dh.fill_mixture(5, d, n, 4, 0.3, labels=True)
+    y = y if labels else None
+    Om = dh.fill_normal(6, n, l)
+    Xd, Omd, yd = dev_pts(X), dh.to_dev(Om), dev_vec(y)
+    errs = {}
+    for prec in (dh.DH_PREC_F64, dh.DH_PREC_BF16):
+        Q = torch.empty((l, n), dtype=torch.float64, device="cuda")
+        W = torch.empty((l, n), dtype=torch.float64, device="cuda")
+        assert dev.lra(Xd, Omd, 1.0 / d, q, Q, W, yd, prec=prec) == 0
+        QtQ = (Q @ Q.T).cpu().numpy()
+        assert np.linalg.norm(QtQ - np.eye(l)) < 1e-12
+        e2, k2 = dev.lra_error(Xd, 1.0 / d, Q, W, yd)
+        errs[prec] = np.sqrt(e2 / k2)
+    Qo, Wo, _ = orc.lra(X, Om, 1.0 / d, q, y)
+    eo, ko = orc.lra_error(X, 1.0 / d, Qo, Wo, y)
+    assert abs(errs[dh.DH_PREC_F64] - np.sqrt(eo / ko)) <= 1e-6 * np.sqrt(eo / ko) + 1e-12
+    assert errs[dh.DH_PREC_BF16] <= 1.5 * errs[dh.DH_PREC_F64] + 3e-3

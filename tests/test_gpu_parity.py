@@ -272,10 +272,27 @@ def test_cholqr2_tall(dev):
 
 
 def test_cholqr2_breakdown(dev):
+    # Exactly dependent columns: every CholeskyQR variant must report breakdown.
     rng = np.random.default_rng(17)
     Y = rng.standard_normal((500, 10))
     Y[:, 7] = Y[:, 2]
     assert dev.cholqr2(dev_mat(Y)) > 0
+
+
+def test_cholqr_shifted_fallback_ill_conditioned(dev):
+    # kappa(Y) = 1e11 is beyond CholeskyQR2 (~u^{-1/2}); the shifted
+    # CholeskyQR3 fallback must still return an orthonormal Q with QR = Y.
+    import torch
+    rng = np.random.default_rng(18)
+    U, _ = np.linalg.qr(rng.standard_normal((4000, 30)))
+    V, _ = np.linalg.qr(rng.standard_normal((30, 30)))
+    Y = U @ np.diag(np.logspace(0, -11, 30)) @ V.T
+    Yd = dev_mat(Y)
+    R = empty_mat(30, 30, torch.float64)
+    assert dev.cholqr2(Yd, R) == 0
+    Q, Rh = dh.from_dev(Yd), dh.from_dev(R)
+    assert np.linalg.norm(Q.T @ Q - np.eye(30)) < 1e-12
+    assert rel(Q @ Rh, Y) < 1e-13
 
 
 # ------------------------------------------------------------------ LRA
