@@ -48,6 +48,42 @@ def cpu_baseline(X, Om, gamma, rows, threads):
                       f"FP64 oracle, {dt:.2f} s"}
 
 
+def core_solve(h, Xd, gamma, nc, nrhs, stream):
+    """Regularised core A = K[:nc,:nc] + 1e-3 I, 64 right-hand sides ~ N(0,1):
+    FP64 blocked Cholesky (bs 256) + two TRSMs, and FP32 Cholesky + FP64
+    iterative refinement to a normwise backward error of 1e-10."""
+    import torch
+    import darray_optimizer_amd as dh
+    A = torch.empty((nc, nc), dtype=torch.float64, device="cuda")
+    Xc = Xd[:nc]
+    h.rbf_tile(Xc, Xc, gamma, A)
+    A.diagonal().add_(1e-3)
+    B = dh.to_dev(np.asfortranarray(dh.fill_normal(2, nc, nrhs)))
+    flops = nc**3 / 3.0 + 2.0 * nc * nc * nrhs
+    out = {"n": nc, "nrhs": nrhs, "lambda": 1e-3, "bs": 256}
+    A64 = A.clone()
+    X64 = B.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = h.posv(A64, X64, 256)
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t0
+    assert st == 0, st
+    R = B - (A @ X64.T).T  # torch FP64 reference residual, outside the timed region
+    be64 = float((R.norm(dim=1) / (A.norm() * X64.norm(dim=1) + B.norm(dim=1))).max())
+    out["fp64"] = {"s": t64, "gflops": flops / t64 / 1e9, "backward_err": be64}
+    Xm = torch.zeros_like(B)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    info, iters, berr = h.posv_mixed(A, B, Xm, 256, 1e-10, 200)
+    torch.cuda.synchronize()
+    tm = time.perf_counter() - t0
+    out["mixed"] = {"s": tm, "iters": iters, "backward_err": berr, "info": info,
+                    "speedup_vs_fp64": t64 / tm}
+    del A, A64
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,6 +97,8 @@ def main(argv=None) -> int:
     ap.add_argument("--q", type=int, default=2)
     ap.add_argument("--cpu-rows", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--core", type=int, default=16384, help="core order (0: skip)")
+    ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--no-error", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1:
@@ -148,6 +186,8 @@ def main(argv=None) -> int:
     }
     if "rel_err" in mixed:
         line["err_ratio_vs_fp64"] = mixed["rel_err"] / fp64["rel_err"]
+    if a.core > 0:
+        line["core_solve"] = core_solve(h, Xd, gamma, a.core, a.nrhs, stream)
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(X, Om, gamma, a.cpu_rows, threads)

@@ -146,6 +146,21 @@ class Handle:
         fn = "dh_potrf_f64" if A.dtype == torch.float64 else "dh_potrf_f32"
         return call(fn, self.h, uplo.encode(), n, _ptr(A), _ld(A), bs)
 
+    def posv(self, A, B, bs=256, uplo="U") -> int:
+        """FP64 Cholesky solve in place: A <- U, B <- A^{-1} B; returns info."""
+        n, nrhs = A.shape[0], B.shape[0]
+        return call("dh_posv_f64", self.h, uplo.encode(), n, nrhs, _ptr(A), _ld(A), _ptr(B),
+                    _ld(B), bs)
+
+    def posv_mixed(self, A, B, X, bs=256, tol=1e-10, max_iter=50, uplo="U"):
+        """FP32 Cholesky + FP64 refinement: X = A^{-1} B; returns (info, iters, berr)."""
+        n, nrhs = A.shape[0], B.shape[0]
+        it = C.c_int(0)
+        be = C.c_double(0.0)
+        info = call("dh_posv_mixed", self.h, uplo.encode(), n, nrhs, _ptr(A), _ld(A), _ptr(B),
+                    _ld(B), _ptr(X), _ld(X), bs, tol, max_iter, C.byref(it), C.byref(be))
+        return info, it.value, be.value
+
     def cholqr2(self, Y, R=None) -> int:
         m, n = Y.shape[1], Y.shape[0]
         return call("dh_cholqr2_f64", self.h, m, n, _ptr(Y), _ld(Y), _ptr(R),

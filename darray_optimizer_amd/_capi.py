@@ -35,6 +35,9 @@ SIGNATURES: dict[str, tuple] = {
     "dh_trsm_f32": (_i, [_p, _c, _c, _c, _c, _i, _i, _f, _p, _i, _p, _i]),
     "dh_potrf_f64": (_i, [_p, _c, _i, _p, _i, _i]),
     "dh_potrf_f32": (_i, [_p, _c, _i, _p, _i, _i]),
+    "dh_posv_f64": (_i, [_p, _c, _i, _i, _p, _i, _p, _i, _i]),
+    "dh_posv_mixed": (_i, [_p, _c, _i, _i, _p, _i, _p, _i, _p, _i, _i, _d, _i, C.POINTER(_i),
+                           C.POINTER(_d)]),
     "dh_cholqr2_f64": (_i, [_p, _i, _i, _p, _i, _p, _i]),
     "dh_lra": (_i, [_p, _i, _i, _i, _i, _i, _p, _i, _p, _d, _p, _i, _p, _i, _p, _i]),
     "dh_lra_error": (_i, [_p, _i, _i, _i, _p, _i, _p, _d, _p, _i, _p, _i, C.POINTER(_d)]),

@@ -21,7 +21,7 @@
 // reduces.  uplo_upper skips tiles strictly below the diagonal and never
 // writes an element with i > j (SYRK semantics).  `stop` (nullable) is a
 // device status word: nonzero makes the kernel return at once.
-template <typename T, int BM, int BN, bool TA, bool TB>
+template <typename T, int BM, int BN, bool TA, bool TB, bool SYMA = false>
 __global__ __launch_bounds__(256) void gemm_kernel(int m, int n, int k, T alpha,
                                                     const T* __restrict__ A, int lda,
                                                     const T* __restrict__ B, int ldb, T beta,
@@ -38,7 +38,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(int m, int n, int k, T alpha,
     const int kend = min(k, kbeg + kchunk);
     typename Mfma<T>::acc_t acc[FM][FN];
     zero_acc<T, FM, FN>(acc);
-    tile_mma<T, BM, BN, TA, TB>(acc, sA, sB, m, n, m0, n0, kbeg, kend, A, lda, B, ldb);
+    tile_mma<T, BM, BN, TA, TB, SYMA>(acc, sA, sB, m, n, m0, n0, kbeg, kend, A, lda, B, ldb);
 #pragma unroll
     for (int a = 0; a < FM; ++a)
 #pragma unroll
@@ -83,11 +83,16 @@ static int launch_gemm(dh_handle_t h, bool ta, bool tb, int m, int n, int k, T a
                        const int* stop) {
     int tm = dh_cdiv(m, BM), tn = dh_cdiv(n, BN);
     long tiles = (long)tm * tn;
+    if (uplo) {  // tiles strictly below the diagonal return at once
+        long below = 0;
+        for (int a = 0; a < tm; ++a) below += max(0, min(tn, (a * BM) / BN));
+        tiles -= below;
+    }
     int nsplit = 1;
     // Tall-skinny products (CholeskyQR Gram, TRSM updates with short outputs)
-    // have few output tiles and a long k: split k to fill the chip.
-    if (tiles < 2L * h->num_cu && k >= 1024) {
-        nsplit = (int)((2L * h->num_cu + tiles - 1) / tiles);
+    // have few output tiles and a long k: split k for ~4 workgroups per CU.
+    if (tiles < 4L * h->num_cu && k >= 1024) {
+        nsplit = (int)((4L * h->num_cu + tiles - 1) / tiles);
         nsplit = min(nsplit, k / 256);
         nsplit = max(nsplit, 1);
         nsplit = min(nsplit, 64);
@@ -148,7 +153,9 @@ static int gemm_stop(dh_handle_t h, bool ta, bool tb, int m, int n, int k, T alp
         DH_CHECK_LAUNCH();
         return 0;
     }
-    if ((long)m * n >= 2048L * 2048L)
+    // FP64 128 x 128 tiles need ~360 registers per lane (one wave per SIMD);
+    // 64 x 64 keeps four waves per SIMD resident.
+    if (sizeof(T) == 4 && (long)m * n >= 2048L * 2048L)
         return launch_gemm<T, 128, 128>(h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
                                         uplo, stop);
     return launch_gemm<T, 64, 64>(h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, uplo,
@@ -436,6 +443,21 @@ int dh_potrf_impl(dh_handle_t h, int n, T* A, int lda, int bs, int* info_host) {
 }
 template int dh_potrf_impl<double>(dh_handle_t, int, double*, int, int, int*);
 template int dh_potrf_impl<float>(dh_handle_t, int, float*, int, int, int*);
+
+// C = alpha * A * B + beta * C with A (m x m) symmetric, upper triangle stored.
+template <typename T>
+int dh_symm_upper_impl(dh_handle_t h, int m, int n, T alpha, const T* A, int lda, const T* B,
+                       int ldb, T beta, T* C, int ldc) {
+    if (m == 0 || n == 0) return 0;
+    constexpr int BM = 64, BN = 64;
+    dim3 grid(dh_cdiv(m, BM), dh_cdiv(n, BN), 1);
+    gemm_kernel<T, BM, BN, false, false, true><<<grid, 256, 0, h->stream>>>(
+        m, n, m, alpha, A, lda, B, ldb, beta, C, ldc, m, 0, nullptr, nullptr);
+    DH_CHECK_LAUNCH();
+    return 0;
+}
+template int dh_symm_upper_impl<double>(dh_handle_t, int, int, double, const double*, int,
+                                        const double*, int, double, double*, int);
 
 // ------------------------------------------------------------- C ABI
 

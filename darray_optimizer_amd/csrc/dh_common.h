@@ -22,6 +22,8 @@ enum dh_slot {
     DH_WS_LOWP_Y,      // FP32 output of a low-precision pass
     DH_WS_PARTIAL,     // per-workgroup partial sums
     DH_WS_INFO,        // device status words
+    DH_WS_SOLVE_A,     // low-precision copy of a matrix being factored
+    DH_WS_SOLVE_R,     // residual / correction panels of iterative refinement
     DH_WS_COUNT
 };
 
@@ -69,3 +71,6 @@ int dh_trsm_impl(dh_handle_t h, char side, char trans, bool unit, int m, int n, 
                  const T* A, int lda, T* B, int ldb);
 template <typename T>
 int dh_potrf_impl(dh_handle_t h, int n, T* A, int lda, int bs, int* info_host);
+template <typename T>
+int dh_symm_upper_impl(dh_handle_t h, int m, int n, T alpha, const T* A, int lda, const T* B,
+                       int ldb, T beta, T* C, int ldc);

@@ -103,62 +103,112 @@ static int cholqr_sweep(dh_handle_t h, int m, int n, double* Y, int ldy, double*
     return dh_trsm_impl<double>(h, 'R', 'N', false, m, n, 1.0, G, n, Y, ldy);
 }
 
-// CholeskyQR2; if the first Gram matrix is numerically singular, shifted
-// CholeskyQR3 (shifted sweep + two plain sweeps).  R (optional) = product of
-// the sweep factors, last one leftmost.
+int dh_chol_small_max_order();
+int dh_chol_small_launch(dh_handle_t h, int n, double* G, int ldg, double* Dinv, int m_rows,
+                         bool shift, bool rel_check, int* info_dev);
+int dh_trsm_rows_launch(dh_handle_t h, int m, int n, double* Y, int ldy, const double* U,
+                        int ldu, const double* Dinv);
+
+// One CholeskyQR sweep in place for orders <= 544: G = upper(Y^T Y) (SYRK),
+// one-workgroup Cholesky with the relative breakdown test, Y := Y U^{-1} by
+// the row-parallel MFMA triangular solve.
+static int cholqr_sweep_small(dh_handle_t h, int m, int n, double* Y, int ldy, double* G,
+                              double* Dinv, bool shift, int* info) {
+    int* dinfo = (int*)dh_ws(h, DH_WS_INFO, 256);
+    if (!dinfo) return DH_ERR_ALLOC;
+    DH_CHECK_STATUS(
+        dh_gemm_impl<double>(h, true, false, n, n, m, 1.0, Y, ldy, Y, ldy, 0.0, G, n, 1));
+    DH_CHECK_STATUS(dh_chol_small_launch(h, n, G, n, Dinv, m, shift, true, dinfo));
+    DH_CHECK_HIP(hipMemcpyAsync(info, dinfo, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    DH_CHECK_HIP(hipStreamSynchronize(h->stream));
+    if (*info) return 0;
+    return dh_trsm_rows_launch(h, m, n, Y, ldy, G, n, Dinv);
+}
+
+// R accumulation across sweeps: Racc = U_sweep * Racc (both upper).
+static int accumulate_r(dh_handle_t h, int n, const double* G, double* Racc, double* Rtmp,
+                        bool first, double* out, int ldo) {
+    size_t nn = (size_t)n * n;
+    if (first) {
+        triu_copy_kernel<<<dh_cdiv((long)nn, 256), 256, 0, h->stream>>>(n, G, n, Racc);
+        DH_CHECK_LAUNCH();
+        if (out)
+            DH_CHECK_HIP(hipMemcpy2DAsync(out, sizeof(double) * ldo, Racc, sizeof(double) * n,
+                                          sizeof(double) * n, n, hipMemcpyDeviceToDevice,
+                                          h->stream));
+        return 0;
+    }
+    triu_copy_kernel<<<dh_cdiv((long)nn, 256), 256, 0, h->stream>>>(n, G, n, Rtmp);
+    DH_CHECK_LAUNCH();
+    double* t2 = Rtmp + nn;
+    DH_CHECK_STATUS(
+        dh_gemm_impl<double>(h, false, false, n, n, n, 1.0, Rtmp, n, Racc, n, 0.0, t2, n, 0));
+    DH_CHECK_HIP(hipMemcpyAsync(Racc, t2, sizeof(double) * nn, hipMemcpyDeviceToDevice,
+                                h->stream));
+    if (out)
+        DH_CHECK_HIP(hipMemcpy2DAsync(out, sizeof(double) * ldo, Racc, sizeof(double) * n,
+                                      sizeof(double) * n, n, hipMemcpyDeviceToDevice, h->stream));
+    return 0;
+}
+
+// CholeskyQR2 (shifted CholeskyQR3 if the first Gram matrix is numerically
+// singular).  Orders up to dh_chol_small_max_order() use the one-workgroup
+// Cholesky and the row-parallel solve; larger orders use the blocked potrf
+// and the recursive TRSM.
 int dh_cholqr2_impl(dh_handle_t h, int m, int n, double* Y, int ldy, double* R, int ldr) {
     if (n == 0) return 0;
     size_t nn = (size_t)n * n;
-    double* G = (double*)dh_ws(h, DH_WS_GRAM, sizeof(double) * nn * 2);
-    if (!G) return DH_ERR_ALLOC;
-    double* Racc = G + nn;
-    double* Rtmp = nullptr;
-    if (R) {
-        Rtmp = (double*)dh_ws(h, DH_WS_RBUF, sizeof(double) * nn);
-        if (!Rtmp) return DH_ERR_ALLOC;
-    }
-    int info = 0;
-    DH_CHECK_STATUS(cholqr_sweep(h, m, n, Y, ldy, G, false, &info));
-    int sweeps = 2;
-    if (info) {
-        // Y is untouched by a failed sweep: restart with the shifted Gram.
-        DH_CHECK_STATUS(cholqr_sweep(h, m, n, Y, ldy, G, true, &info));
-        if (info)
-            return dh_fail(info, "CholeskyQR: shifted Gram matrix not positive definite (pivot %d)",
-                           info);
-        sweeps = 3;
-    }
-    for (int sw = 1; sw <= sweeps; ++sw) {
-        if (sw > 1) {
+    if (n > dh_chol_small_max_order()) {
+        double* G = (double*)dh_ws(h, DH_WS_GRAM, sizeof(double) * nn * 4);
+        if (!G) return DH_ERR_ALLOC;
+        double *Racc = G + nn, *Rtmp = G + 2 * nn;
+        int info = 0, sweeps = 2;
+        DH_CHECK_STATUS(cholqr_sweep(h, m, n, Y, ldy, G, false, &info));
+        if (info) {
+            DH_CHECK_STATUS(cholqr_sweep(h, m, n, Y, ldy, G, true, &info));
+            if (info)
+                return dh_fail(info, "CholeskyQR: shifted Gram matrix not positive definite "
+                                     "(pivot %d)", info);
+            sweeps = 3;
+        }
+        if (R) DH_CHECK_STATUS(accumulate_r(h, n, G, Racc, Rtmp, true, sweeps == 1 ? R : nullptr, ldr));
+        for (int sw = 2; sw <= sweeps; ++sw) {
             DH_CHECK_STATUS(cholqr_sweep(h, m, n, Y, ldy, G, false, &info));
             if (info)
                 return dh_fail(1000000 * (sw - 1) + info,
                                "CholeskyQR: Gram matrix of sweep %d is numerically singular "
                                "(pivot %d)", sw, info);
+            if (R)
+                DH_CHECK_STATUS(accumulate_r(h, n, G, Racc, Rtmp, false,
+                                             sw == sweeps ? R : nullptr, ldr));
         }
-        if (R) {
-            if (sw == 1) {
-                triu_copy_kernel<<<dh_cdiv((long)nn, 256), 256, 0, h->stream>>>(n, G, n, Racc);
-                DH_CHECK_LAUNCH();
-            } else {
-                triu_copy_kernel<<<dh_cdiv((long)nn, 256), 256, 0, h->stream>>>(n, G, n, Rtmp);
-                DH_CHECK_LAUNCH();
-                double* out = (sw == sweeps) ? R : Racc;
-                int ldo = (sw == sweeps) ? ldr : n;
-                if (sw == sweeps) {
-                    DH_CHECK_STATUS(dh_gemm_impl<double>(h, false, false, n, n, n, 1.0, Rtmp, n,
-                                                         Racc, n, 0.0, out, ldo, 0));
-                } else {
-                    // Racc := Rtmp * Racc via a temporary (GEMM output may not alias).
-                    double* t2 = (double*)dh_ws(h, DH_WS_PARTIAL, sizeof(double) * nn);
-                    if (!t2) return DH_ERR_ALLOC;
-                    DH_CHECK_STATUS(dh_gemm_impl<double>(h, false, false, n, n, n, 1.0, Rtmp, n,
-                                                         Racc, n, 0.0, t2, n, 0));
-                    DH_CHECK_HIP(hipMemcpyAsync(Racc, t2, sizeof(double) * nn,
-                                                hipMemcpyDeviceToDevice, h->stream));
-                }
-            }
-        }
+        return 0;
+    }
+    double* G = (double*)dh_ws(h, DH_WS_GRAM, sizeof(double) * nn * 5);
+    if (!G) return DH_ERR_ALLOC;
+    double *Racc = G + nn, *Rtmp = G + 2 * nn;  // Rtmp spans 2 nn
+    double* Dbuf = (double*)dh_ws(h, DH_WS_RBUF, sizeof(double) * 256 * (size_t)dh_cdiv(n, 16));
+    if (!Dbuf) return DH_ERR_ALLOC;
+    int info = 0, sweeps = 2;
+    DH_CHECK_STATUS(cholqr_sweep_small(h, m, n, Y, ldy, G, Dbuf, false, &info));
+    if (info) {
+        // A failed sweep leaves Y untouched: restart with the shifted Gram.
+        DH_CHECK_STATUS(cholqr_sweep_small(h, m, n, Y, ldy, G, Dbuf, true, &info));
+        if (info)
+            return dh_fail(info, "CholeskyQR: shifted Gram matrix not positive definite (pivot %d)",
+                           info);
+        sweeps = 3;
+    }
+    if (R) DH_CHECK_STATUS(accumulate_r(h, n, G, Racc, Rtmp, true, nullptr, ldr));
+    for (int sw = 2; sw <= sweeps; ++sw) {
+        DH_CHECK_STATUS(cholqr_sweep_small(h, m, n, Y, ldy, G, Dbuf, false, &info));
+        if (info)
+            return dh_fail(1000000 * (sw - 1) + info,
+                           "CholeskyQR: Gram matrix of sweep %d is numerically singular (pivot %d)",
+                           sw, info);
+        if (R)
+            DH_CHECK_STATUS(
+                accumulate_r(h, n, G, Racc, Rtmp, false, sw == sweeps ? R : nullptr, ldr));
     }
     return 0;
 }

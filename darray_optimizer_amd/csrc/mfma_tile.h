@@ -38,11 +38,13 @@ constexpr int kTilePad = 4;
 // with a 256-thread workgroup: waves in a 2 x 2 grid, each owning a
 // (BM/2) x (BN/2) sub-tile as (BM/32) x (BN/32) MFMA blocks.  K advances in
 // steps of 16 staged through LDS as op(A)^T and op(B) (k-major rows), so each
-// MFMA operand read is 16 consecutive LDS elements.  Out-of-range elements
-// are staged as zero.  Caller provides sA[16][BM+4] and sB[16][BN+4].
+// MFMA operand read is 16 consecutive LDS elements; the global loads of step
+// k+1 are issued before the MFMAs of step k.  Out-of-range elements are
+// staged as zero.  Caller provides sA[16][BM+4] and sB[16][BN+4].
 //   op(A)[i][k] = TA ? A[k + i*lda] : A[i + k*lda]
+//                 (SYMA: A symmetric, upper triangle stored, op(A)[i][k] = A[min][max])
 //   op(B)[k][j] = TB ? B[j + k*ldb] : B[k + j*ldb]
-template <typename T, int BM, int BN, bool TA, bool TB>
+template <typename T, int BM, int BN, bool TA, bool TB, bool SYMA = false>
 __device__ __forceinline__ void tile_mma(typename Mfma<T>::acc_t (&acc)[BM / 32][BN / 32],
                                          T (*sA)[BM + kTilePad], T (*sB)[BN + kTilePad], int m,
                                          int n, int m0, int n0, int kbeg, int kend,
@@ -50,37 +52,60 @@ __device__ __forceinline__ void tile_mma(typename Mfma<T>::acc_t (&acc)[BM / 32]
                                          const T* __restrict__ B, int ldb) {
     constexpr int BK = kTileBK;
     constexpr int FM = BM / 32, FN = BN / 32;
+    constexpr int PA = BM * BK / 256, PB = BN * BK / 256;  // staged elements per thread
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int g = lane >> 4, li = lane & 15;
-    for (int k0 = kbeg; k0 < kend; k0 += BK) {
-        if (!TA) {
+    T ra[PA], rb[PB];
+    // Element u of this thread's share of the A / B tile at k-offset k0.
+    auto a_at = [&](int u, int& i, int& kk) {
+        int e = tid + u * 256;
+        if (!TA) { i = e % BM; kk = e / BM; } else { kk = e % BK; i = e / BK; }
+    };
+    auto b_at = [&](int u, int& j, int& kk) {
+        int e = tid + u * 256;
+        if (!TB) { kk = e % BK; j = e / BK; } else { j = e % BN; kk = e / BN; }
+    };
+    auto fetch = [&](int k0) {
 #pragma unroll
-            for (int e = tid; e < BM * BK; e += 256) {
-                int i = e % BM, kk = e / BM, gi = m0 + i, gk = k0 + kk;
-                sA[kk][i] = (gi < m && gk < kend) ? A[gi + (size_t)gk * lda] : T(0);
-            }
-        } else {
-#pragma unroll
-            for (int e = tid; e < BM * BK; e += 256) {
-                int kk = e % BK, i = e / BK, gi = m0 + i, gk = k0 + kk;
-                sA[kk][i] = (gi < m && gk < kend) ? A[gk + (size_t)gi * lda] : T(0);
-            }
+        for (int u = 0; u < PA; ++u) {
+            int i, kk;
+            a_at(u, i, kk);
+            int gi = m0 + i, gk = k0 + kk;
+            // SYMA: A is symmetric with only its upper triangle stored.
+            ra[u] = (gi < m && gk < kend)
+                        ? (SYMA ? (gi <= gk ? A[gi + (size_t)gk * lda] : A[gk + (size_t)gi * lda])
+                                : (TA ? A[gk + (size_t)gi * lda] : A[gi + (size_t)gk * lda]))
+                        : T(0);
         }
-        if (!TB) {
 #pragma unroll
-            for (int e = tid; e < BN * BK; e += 256) {
-                int kk = e % BK, j = e / BK, gj = n0 + j, gk = k0 + kk;
-                sB[kk][j] = (gj < n && gk < kend) ? B[gk + (size_t)gj * ldb] : T(0);
-            }
-        } else {
+        for (int u = 0; u < PB; ++u) {
+            int j, kk;
+            b_at(u, j, kk);
+            int gj = n0 + j, gk = k0 + kk;
+            rb[u] = (gj < n && gk < kend)
+                        ? (TB ? B[gj + (size_t)gk * ldb] : B[gk + (size_t)gj * ldb])
+                        : T(0);
+        }
+    };
+    if (kbeg >= kend) return;
+    fetch(kbeg);
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
 #pragma unroll
-            for (int e = tid; e < BN * BK; e += 256) {
-                int j = e % BN, kk = e / BN, gj = n0 + j, gk = k0 + kk;
-                sB[kk][j] = (gj < n && gk < kend) ? B[gj + (size_t)gk * ldb] : T(0);
-            }
+        for (int u = 0; u < PA; ++u) {
+            int i, kk;
+            a_at(u, i, kk);
+            sA[kk][i] = ra[u];
+        }
+#pragma unroll
+        for (int u = 0; u < PB; ++u) {
+            int j, kk;
+            b_at(u, j, kk);
+            sB[kk][j] = rb[u];
         }
         __syncthreads();
+        // The next tile's global loads are in flight while this one computes.
+        if (k0 + BK < kend) fetch(k0 + BK);
 #pragma unroll
         for (int ks = 0; ks < BK / 4; ++ks) {
             T av[FM], bv[FN];

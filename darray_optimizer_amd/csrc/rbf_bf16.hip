@@ -112,9 +112,11 @@ __global__ __launch_bounds__(256) void pack_omega_kernel(int n, int l, int nchun
 // ------------------------------------------------------------ main kernel
 //
 // Workgroup = WAVES waves; wave w owns 32 rows i (two 16-row MFMA blocks).
-// The j loop walks chunks of 32 points; chunk t+1 is fetched into registers
-// while chunk t is computed, and stored to the other LDS buffer (one barrier
-// per chunk).  Per chunk each wave:
+// The j loop walks chunks of 32 points through a 3-deep LDS ring: while
+// chunk t is projected, chunk t+1 is already resident (its distance tile is
+// computed in the same step) and chunk t+2 is being staged, so one barrier
+// per chunk suffices and the loop body is a single basic block (staging and
+// look-ahead indices are clamped instead of branched).  Per chunk a wave:
 //   1. S^T = Xj_chunk * Xi^T: 2 x 2 blocks of 16x16, DK MFMA 16x16x32 each;
 //      A operand (Xj rows) from LDS, B operand (Xi rows) from registers.  Lane
 //      L holds S^T[16 jb + 4 g + r][16 ib + li] (g = L/16, li = L%16).
@@ -123,6 +125,8 @@ __global__ __launch_bounds__(256) void pack_omega_kernel(int n, int l, int nchun
 //      k-slot 8 g + e <-> j = 16 (e/4) + 4 g + e%4 -- the order Op is packed in;
 //   3. Y[i, c] += K[i, j] Op[j, c]: NCB column blocks x 2 row blocks, B operand
 //      = one 16-byte LDS read per column block shared by both row blocks.
+// Step t runs the epilogue of chunk t next to the distance MFMAs of chunk t+1
+// (independent work the scheduler interleaves), then the projection of t.
 // Accumulators stay in registers for the whole j loop; Y is written once.
 template <int DK, int NCB, int WAVES>
 __global__ __launch_bounds__(WAVES * 64, 1) void rbf_project_bf16_kernel(
@@ -137,9 +141,10 @@ __global__ __launch_bounds__(WAVES * 64, 1) void rbf_project_bf16_kernel(
     constexpr int OGRP = NCB * 64; // groups of Op per chunk (this column range)
     constexpr int XPT = (XGRP + NT - 1) / NT;
     constexpr int OPT = (OGRP + NT - 1) / NT;
-    __shared__ bf16x8 sX[2][32 * XROW];
-    __shared__ bf16x8 sO[2][OGRP];
-    __shared__ float sB[2][32];
+    constexpr int NBUF = 3;
+    __shared__ bf16x8 sX[NBUF][32 * XROW];
+    __shared__ bf16x8 sO[NBUF][OGRP];
+    __shared__ float sB[NBUF][32];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, li = lane & 15;
@@ -170,12 +175,12 @@ __global__ __launch_bounds__(WAVES * 64, 1) void rbf_project_bf16_kernel(
 #pragma unroll
         for (int u = 0; u < XPT; ++u) {
             int e = tid + u * NT;
-            if (e < XGRP) rx[u] = Xj[(size_t)t * XGRP + e];
+            if (XGRP % NT == 0 || e < XGRP) rx[u] = Xj[(size_t)t * XGRP + e];
         }
 #pragma unroll
         for (int u = 0; u < OPT; ++u) {
             int e = tid + u * NT;
-            if (e < OGRP) ro[u] = Op[((size_t)t * ncb_total + cb0) * 64 + e];
+            if (OGRP % NT == 0 || e < OGRP) ro[u] = Op[((size_t)t * ncb_total + cb0) * 64 + e];
         }
         if (tid < 32) rb = bj[t * 32 + tid];
     };
@@ -183,24 +188,16 @@ __global__ __launch_bounds__(WAVES * 64, 1) void rbf_project_bf16_kernel(
 #pragma unroll
         for (int u = 0; u < XPT; ++u) {
             int e = tid + u * NT;
-            if (e < XGRP) sX[buf][(e / XG) * XROW + (e % XG)] = rx[u];
+            if (XGRP % NT == 0 || e < XGRP) sX[buf][(e / XG) * XROW + (e % XG)] = rx[u];
         }
 #pragma unroll
         for (int u = 0; u < OPT; ++u) {
             int e = tid + u * NT;
-            if (e < OGRP) sO[buf][e] = ro[u];
+            if (OGRP % NT == 0 || e < OGRP) sO[buf][e] = ro[u];
         }
         if (tid < 32) sB[buf][tid] = rb;
     };
-
-    gload(0);
-    sstore(0);
-    __syncthreads();
-    for (int t = 0; t < nchunks; ++t) {
-        const int buf = t & 1;
-        if (t + 1 < nchunks) gload(t + 1);
-
-        f32x4 s[2][2];
+    auto distance = [&](int buf, f32x4 (&s)[2][2]) {
 #pragma unroll
         for (int jb = 0; jb < 2; ++jb)
 #pragma unroll
@@ -213,6 +210,22 @@ __global__ __launch_bounds__(WAVES * 64, 1) void rbf_project_bf16_kernel(
 #pragma unroll
                 for (int ib = 0; ib < 2; ++ib) s[jb][ib] = mfma_bf16(a, xb[ib][ks], s[jb][ib]);
             }
+    };
+
+    // Prologue: chunks 0 and 1 resident, distance tile of chunk 0 computed.
+    const int last = nchunks - 1;
+    gload(0);
+    sstore(0);
+    gload(min(1, last));
+    sstore(1);
+    __syncthreads();
+    f32x4 s_cur[2][2];
+    distance(0, s_cur);
+    int buf = 0;  // buffer of chunk t
+    for (int t = 0; t < nchunks; ++t) {
+        const int nbuf = (buf == NBUF - 1) ? 0 : buf + 1;     // chunk t+1
+        const int sbuf = (nbuf == NBUF - 1) ? 0 : nbuf + 1;   // chunk t+2 (being staged)
+        gload(min(t + 2, last));
 
         bf16x8 kb[2];
 #pragma unroll
@@ -222,10 +235,12 @@ __global__ __launch_bounds__(WAVES * 64, 1) void rbf_project_bf16_kernel(
                 float bv = sB[buf][16 * jb + 4 * g + r];
 #pragma unroll
                 for (int ib = 0; ib < 2; ++ib) {
-                    float arg = fmaf(c2, s[jb][ib][r], arow[ib] + bv);
+                    float arg = fmaf(c2, s_cur[jb][ib][r], arow[ib] + bv);
                     kb[ib][4 * jb + r] = (__bf16)__builtin_amdgcn_exp2f(fminf(arg, 0.f));
                 }
             }
+        f32x4 s_next[2][2];
+        distance(nbuf, s_next);
 
 #pragma unroll
         for (int cb = 0; cb < NCB; ++cb) {
@@ -233,8 +248,13 @@ __global__ __launch_bounds__(WAVES * 64, 1) void rbf_project_bf16_kernel(
 #pragma unroll
             for (int ib = 0; ib < 2; ++ib) acc[ib][cb] = mfma_bf16(kb[ib], bo, acc[ib][cb]);
         }
-        if (t + 1 < nchunks) sstore(buf ^ 1);
+        sstore(sbuf);
         __syncthreads();
+#pragma unroll
+        for (int jb = 0; jb < 2; ++jb)
+#pragma unroll
+            for (int ib = 0; ib < 2; ++ib) s_cur[jb][ib] = s_next[jb][ib];
+        buf = nbuf;
     }
 
 #pragma unroll
@@ -253,14 +273,18 @@ __global__ __launch_bounds__(WAVES * 64, 1) void rbf_project_bf16_kernel(
             }
 }
 
-constexpr int kWaves = 8;
-constexpr int kRowsPerWG = kWaves * 32;
+// Waves per workgroup: 8 (two per SIMD, 256 registers each) while the Xi
+// fragments are small; 4 (one per SIMD, 512 registers) for d > 128.
+template <int DK>
+constexpr int waves_for() { return DK >= 8 ? 4 : 8; }
 
 template <int DK, int NCB>
-void launch_bf16(dh_handle_t h, dim3 grid, int m, int nchunks, int l, int ncb_total,
+void launch_bf16(dh_handle_t h, int m, int groups, int nchunks, int l, int ncb_total,
                  const bf16x8* Xi, const bf16x8* Xj, const float* ai, const float* bj, float c2,
                  const bf16x8* Op, const double* yi, double* Y, int ldy) {
-    rbf_project_bf16_kernel<DK, NCB, kWaves><<<grid, kWaves * 64, 0, h->stream>>>(
+    constexpr int W = waves_for<DK>();
+    dim3 grid(dh_cdiv(m, W * 32), groups);
+    rbf_project_bf16_kernel<DK, NCB, W><<<grid, W * 64, 0, h->stream>>>(
         m, nchunks, l, ncb_total, Xi, Xj, ai, bj, c2, Op, yi, Y, ldy);
 }
 
@@ -272,13 +296,13 @@ int ncb_instance_bf16(int want) {
 }
 
 template <int DK>
-int dispatch_bf16_ncb(dh_handle_t h, int ncb, dim3 grid, int m, int nchunks, int l,
+int dispatch_bf16_ncb(dh_handle_t h, int ncb, int groups, int m, int nchunks, int l,
                       int ncb_total, const bf16x8* Xi, const bf16x8* Xj, const float* ai,
                       const float* bj, float c2, const bf16x8* Op, const double* yi, double* Y,
                       int ldy) {
 #define DH_BF16_NCB(N)                                                                        \
     case N:                                                                                   \
-        launch_bf16<DK, N>(h, grid, m, nchunks, l, ncb_total, Xi, Xj, ai, bj, c2, Op, yi, Y, ldy); \
+        launch_bf16<DK, N>(h, m, groups, nchunks, l, ncb_total, Xi, Xj, ai, bj, c2, Op, yi, Y, ldy); \
         break;
     switch (ncb) {
         DH_BF16_NCB(1)
@@ -366,7 +390,7 @@ int project_bf16(dh_handle_t h, int m, int n, int d, int l, const LowpPoints& pi
     int nchunks = dh_cdiv(n, 32);
     if (nchunks == 0) return dh_fail(DH_ERR_UNSUPPORTED, "rbf_project_bf16: n = 0");
     Bf16Cols cols = bf16_cols(l);
-    dim3 grid(dh_cdiv(m, kRowsPerWG), cols.groups);
+    const int grid = cols.groups;
     const bf16x8* xi = (const bf16x8*)pi.xb;
     const bf16x8* xj = (const bf16x8*)pj.xb;
     switch (dp) {

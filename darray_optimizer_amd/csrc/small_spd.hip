@@ -1,0 +1,359 @@
+// The k x k step of CholeskyQR on gfx950, and the row-parallel triangular
+// solve that applies its factor to the n x k panel.
+//
+// chol_small_kernel (one workgroup, order n <= 544): G (upper triangle read)
+// is overwritten by U with G = U^T U; the strict lower triangle is untouched.
+// Right-looking in 16-column panels as cholesky_factorize (lapack_like.h:17-52):
+// factor the diagonal block (one wave, column per lane), solve the row panel
+// U12 = U11^{-T} A12 (column per thread), update A22 -= U12^T U12 in 64 x 64
+// blocks (wave per block, row per lane, panel rows cached in registers).
+// It also writes the inverses of the 16 x 16 diagonal blocks of U to Dinv
+// (block b at Dinv + 256 b, column-major), for trsm_rows_kernel.
+//
+// Breakdown: a pivot p (before the square root) fails if !(p > tol) with
+// tol = 0 (plain Cholesky) or tol = n * eps * max_k G_kk (CholeskyQR: Y
+// numerically rank deficient); *info receives the 1-based index of the first
+// failing pivot, or 0.  With shift != 0 the diagonal is first raised by
+// s = 11 (m n + n (n+1)) u trace(G) (shifted CholeskyQR3, Fukaya et al. 2020).
+//
+// trsm_rows_kernel: Q = Y U^{-1} in place, one wave per 16 rows of Y.  Rows
+// are independent, so each wave runs the whole block-column recurrence
+//   Q_J = (Y_J - sum_{K<J} Q_K U_KJ) U_JJ^{-1}
+// for its rows on FP64 MFMA, its solved rows staged in LDS.
+#include "dh_common.h"
+#include "mfma_tile.h"
+
+namespace {
+
+constexpr int NB = 16;
+constexpr int kMaxN = 544;
+constexpr int PW = kMaxN - NB + 4;  // LDS panel pitch, multiple of 4 doubles
+constexpr int NT = 512;
+
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// Orders LDS traffic of one wave (no workgroup barrier: waves are independent).
+// Waits for LDS traffic only (s_waitcnt lgkmcnt(0) with vmcnt / expcnt left at
+// their maxima), so prefetched global loads stay in flight across it.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    __builtin_amdgcn_wave_barrier();
+}
+
+__global__ __launch_bounds__(NT) void chol_small_kernel(int n, double* __restrict__ G, int ldg,
+                                                        double* __restrict__ Dinv, int m_rows,
+                                                        int shift, int rel_check,
+                                                        int* __restrict__ info) {
+    __shared__ double sD[NB][NB + 1];            // diagonal block, sD[j][i] = A(k0+i, k0+j)
+    __shared__ __align__(16) double sP[NB][PW];  // row panel U12, sP[kk][c]
+    __shared__ double red[2][NT / 64];
+    __shared__ double s_tol, s_shift;
+    __shared__ int s_info;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int NW = NT / 64;
+
+    {
+        double mx = 0.0, tr = 0.0;
+        for (int k = tid; k < n; k += NT) {
+            double v = G[k + (size_t)k * ldg];
+            mx = fmax(mx, v);
+            tr += v;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            mx = fmax(mx, __shfl_xor(mx, o));
+            tr += __shfl_xor(tr, o);
+        }
+        if (lane == 0) {
+            red[0][wave] = mx;
+            red[1][wave] = tr;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double m2 = 0.0, t2 = 0.0;
+            for (int w = 0; w < NW; ++w) {
+                m2 = fmax(m2, red[0][w]);
+                t2 += red[1][w];
+            }
+            s_tol = rel_check ? (double)n * 2.220446049250313e-16 * m2 : 0.0;
+            s_shift = shift ? 11.0 * ((double)m_rows * n + (double)n * (n + 1)) *
+                                  1.1102230246251565e-16 * t2
+                            : 0.0;
+            s_info = 0;
+        }
+        __syncthreads();
+        if (shift)
+            for (int k = tid; k < n; k += NT) G[k + (size_t)k * ldg] += s_shift;
+        __syncthreads();
+    }
+    const double tol = s_tol;
+
+    for (int k0 = 0; k0 < n; k0 += NB) {
+        const int nb = min(NB, n - k0), w = n - k0 - nb;
+        for (int e = tid; e < nb * nb; e += NT) {
+            int i = e % nb, j = e / nb;
+            sD[j][i] = (i <= j) ? G[(k0 + i) + (size_t)(k0 + j) * ldg] : 0.0;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            // Lane j owns column j of the block; row kk of U is broadcast with readlane.
+            double a[NB];
+#pragma unroll
+            for (int i = 0; i < NB; ++i) a[i] = (lane < nb) ? sD[lane][i] : 0.0;
+            int bad = 0;
+#pragma unroll
+            for (int kk = 0; kk < NB; ++kk) {
+                if (kk < nb && !bad) {
+                    double piv = readlane_d(a[kk], kk);
+                    if (!(piv > tol)) {
+                        bad = kk + 1;
+                    } else {
+                        double u = sqrt(piv);
+                        double ukj = (lane == kk) ? u : a[kk] / u;
+                        a[kk] = (lane >= kk) ? ukj : a[kk];
+#pragma unroll
+                        for (int i = kk + 1; i < NB; ++i) {
+                            double uki = readlane_d(ukj, i);
+                            if (lane > kk && i <= lane && lane < nb) a[i] -= uki * ukj;
+                        }
+                    }
+                }
+            }
+            if (lane < nb)
+#pragma unroll
+                for (int i = 0; i < NB; ++i)
+                    if (i <= lane) sD[lane][i] = a[i];
+            if (lane == 0 && bad) s_info = k0 + bad;
+        }
+        __syncthreads();
+        if (s_info) break;
+        for (int e = tid; e < nb * nb; e += NT) {
+            int i = e % nb, j = e / nb;
+            if (i <= j) G[(k0 + i) + (size_t)(k0 + j) * ldg] = sD[j][i];
+        }
+        if (w == 0) break;  // last panel; nb == NB whenever w > 0
+        // Row panel U12 = U11^{-T} A12, one column per thread.
+        for (int t = tid; t < w; t += NT) {
+            // Keep the LDS reads of U11 inside the loop (hoisting all NB^2 of
+            // them out of it exhausts the register file).
+            asm volatile("" ::: "memory");
+            const int c = k0 + NB + t;
+            double x[NB];
+#pragma unroll
+            for (int kk = 0; kk < NB; ++kk) x[kk] = G[(k0 + kk) + (size_t)c * ldg];
+#pragma unroll
+            for (int kk = 0; kk < NB; ++kk) {
+                double s = x[kk];
+#pragma unroll
+                for (int p = 0; p < kk; ++p) s -= sD[kk][p] * x[p];
+                x[kk] = s / sD[kk][kk];
+            }
+#pragma unroll
+            for (int kk = 0; kk < NB; ++kk) {
+                G[(k0 + kk) + (size_t)c * ldg] = x[kk];
+                sP[kk][t] = x[kk];
+            }
+        }
+        __syncthreads();
+        // A22 -= U12^T U12 on the upper triangle: 64 x 64 blocks, wave per block,
+        // lane = row (its NB panel values in registers), four columns per step.
+        const int wb = (w + 63) / 64;
+        const int nblk = wb * (wb + 1) / 2;
+        for (int blk = wave; blk < nblk; blk += NW) {
+            int bi = 0, rem = blk;
+            while (rem >= wb - bi) {
+                rem -= wb - bi;
+                ++bi;
+            }
+            const int bj = bi + rem;
+            const int i = 64 * bi + lane;
+            double u[NB];
+#pragma unroll
+            for (int kk = 0; kk < NB; ++kk) u[kk] = (i < w) ? sP[kk][i] : 0.0;
+            const int jend = min(64 * bj + 64, w);
+            for (int j0 = 64 * bj; j0 < jend; j0 += 4) {
+                // Issue the loads of the four target elements before the FMAs.
+                double gv[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    int j = j0 + q;
+                    gv[q] = (i < w && j < w && i <= j)
+                                ? G[(k0 + NB + i) + (size_t)(k0 + NB + j) * ldg]
+                                : 0.0;
+                }
+                double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+                for (int kk = 0; kk < NB; ++kk) {
+                    const double2* row = reinterpret_cast<const double2*>(&sP[kk][j0]);
+                    double2 p01 = row[0], p23 = row[1];
+                    s0 += u[kk] * p01.x;
+                    s1 += u[kk] * p01.y;
+                    s2 += u[kk] * p23.x;
+                    s3 += u[kk] * p23.y;
+                }
+                double s[4] = {s0, s1, s2, s3};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    int j = j0 + q;
+                    if (i < w && j < w && i <= j)
+                        G[(k0 + NB + i) + (size_t)(k0 + NB + j) * ldg] = gv[q] - s[q];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (s_info) {
+        if (tid == 0) *info = s_info;
+        return;
+    }
+    __syncthreads();
+    // Inverses of the 16 x 16 diagonal blocks: wave per block, lane per column.
+    const int nb16 = (n + 15) / 16;
+    for (int b = wave; b < nb16; b += NW) {
+        const int c0 = 16 * b, nb = min(16, n - c0);
+        // Lane j < 16 loads column j of the block, then rows are broadcast
+        // from lane k with readlane: U[i][k] = column k, row i.
+        double col[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            col[i] = (lane < nb && i < nb) ? G[(c0 + i) + (size_t)(c0 + lane) * ldg] : 0.0;
+        double x[16];
+#pragma unroll
+        for (int i = 15; i >= 0; --i) {
+            double s = (i == lane) ? 1.0 : 0.0;
+            if (i < nb) {
+#pragma unroll
+                for (int k = i + 1; k < 16; ++k)
+                    if (k < nb) s -= readlane_d(col[i], k) * x[k];
+                x[i] = s / readlane_d(col[i], i);
+            } else {
+                x[i] = 0.0;
+            }
+        }
+        if (lane < 16)
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                Dinv[256 * b + i + 16 * lane] = (i <= lane && lane < nb) ? x[i] : 0.0;
+    }
+    if (tid == 0) *info = 0;
+}
+
+// Q = Y U^{-1} in place for the m x n panel Y (n <= 544), one wave per 16 rows.
+// The wave keeps its 16 solved rows in LDS (odd row pitch: conflict-free
+// A-operand reads) and for each 16-column block J accumulates
+//   acc = sum_{K<J} Q[:,K] U[K,J]      (f64 MFMA 16x16x4 steps, U from L2)
+// then Q_J = (Y_J - acc) U_JJ^{-1} (four more MFMAs, U_JJ^{-1} from Dinv).
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void trsm_rows_kernel(int m, int n,
+                                                               double* __restrict__ Y, int ldy,
+                                                               const double* __restrict__ U,
+                                                               int ldu,
+                                                               const double* __restrict__ Dinv,
+                                                               int pitch) {
+    extern __shared__ double smem[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, li = lane & 15;
+    double* Qs = smem + (size_t)wave * 16 * pitch;  // Qs[row * pitch + col]
+    const int r0 = (blockIdx.x * WAVES + wave) * 16;
+    if (r0 >= m) return;
+    const int nblk = (n + 15) / 16;
+    for (int J = 0; J < nblk; ++J) {
+        const int c0 = 16 * J, col = c0 + li;
+        const bool cok = col < n;
+        // Prefetch this block's Y values and U_JJ^{-1} column before the k loop.
+        double yv[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int row = r0 + g + 4 * r;
+            yv[r] = (row < m && cok) ? Y[row + (size_t)col * ldy] : 0.0;
+        }
+        const double* Di = Dinv + 256 * J;
+        double dv[4];
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) dv[ks] = Di[(4 * ks + g) + 16 * li];  // U_JJ^{-1}[4ks+g][li]
+        // acc = sum_{K<J} Q[:,K] U[K,J]: four independent accumulator chains per
+        // k-step of 16, U loads software-pipelined one step ahead.
+        dh_d4 acc0 = {0, 0, 0, 0}, acc1 = acc0, acc2 = acc0, acc3 = acc0;
+        const double* Ucol = U + (size_t)(cok ? col : 0) * ldu;
+        const double cm = cok ? 1.0 : 0.0;
+        double b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+        if (c0 > 0) {
+            b0 = Ucol[g];
+            b1 = Ucol[4 + g];
+            b2 = Ucol[8 + g];
+            b3 = Ucol[12 + g];
+        }
+        for (int k = 0; k < c0; k += 16) {
+            double c0v = b0, c1v = b1, c2v = b2, c3v = b3;
+            if (k + 16 < c0) {
+                b0 = Ucol[k + 16 + g];
+                b1 = Ucol[k + 20 + g];
+                b2 = Ucol[k + 24 + g];
+                b3 = Ucol[k + 28 + g];
+            }
+            const double* qa = Qs + li * pitch + k + g;  // Q[r0+li][k+g+4s]
+            acc0 = Mfma<double>::op(qa[0], c0v * cm, acc0);
+            acc1 = Mfma<double>::op(qa[4], c1v * cm, acc1);
+            acc2 = Mfma<double>::op(qa[8], c2v * cm, acc2);
+            acc3 = Mfma<double>::op(qa[12], c3v * cm, acc3);
+        }
+        dh_d4 acc = (acc0 + acc1) + (acc2 + acc3);
+        // T = Y_J - acc; accumulator r of lane L is row g + 4r, column li.
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Qs[(g + 4 * r) * pitch + col] = yv[r] - acc[r];
+        wave_lds_sync();
+        dh_d4 q = {0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+            q = Mfma<double>::op(Qs[li * pitch + c0 + 4 * ks + g], dv[ks], q);  // T[li][4ks+g]
+        wave_lds_sync();
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int row = r0 + g + 4 * r;
+            Qs[(g + 4 * r) * pitch + col] = q[r];
+            if (row < m && cok) Y[row + (size_t)col * ldy] = q[r];
+        }
+        wave_lds_sync();
+    }
+}
+
+}  // namespace
+
+int dh_chol_small_max_order() { return kMaxN; }
+
+// One-workgroup Cholesky of G (+ 16 x 16 diagonal-block inverses into Dinv,
+// which needs 256 * ceil(n/16) doubles); *info_dev receives the status.
+int dh_chol_small_launch(dh_handle_t h, int n, double* G, int ldg, double* Dinv, int m_rows,
+                         bool shift, bool rel_check, int* info_dev) {
+    if (n > kMaxN) return dh_fail(DH_ERR_UNSUPPORTED, "chol_small: order %d > %d", n, kMaxN);
+    chol_small_kernel<<<1, NT, 0, h->stream>>>(n, G, ldg, Dinv, m_rows, shift ? 1 : 0,
+                                               rel_check ? 1 : 0, info_dev);
+    DH_CHECK_LAUNCH();
+    return 0;
+}
+
+// Y := Y U^{-1} with U from chol_small (upper, order n <= 544) and its Dinv.
+int dh_trsm_rows_launch(dh_handle_t h, int m, int n, double* Y, int ldy, const double* U,
+                        int ldu, const double* Dinv) {
+    if (n > kMaxN) return dh_fail(DH_ERR_UNSUPPORTED, "trsm_rows: order %d > %d", n, kMaxN);
+    if (m == 0 || n == 0) return 0;
+    constexpr int W = 2;
+    int pitch = 16 * ((n + 15) / 16) + 1;
+    size_t lds = sizeof(double) * 16 * (size_t)pitch * W;
+    static bool lds_attr_set = false;  // allow the full 160 KB of dynamic LDS once
+    if (!lds_attr_set) {
+        DH_CHECK_HIP(hipFuncSetAttribute((const void*)trsm_rows_kernel<W>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)(sizeof(double) * 16 * (16 * 34 + 1) * W)));
+        lds_attr_set = true;
+    }
+    trsm_rows_kernel<W><<<dh_cdiv(m, 16 * W), W * 64, lds, h->stream>>>(m, n, Y, ldy, U, ldu,
+                                                                          Dinv, pitch);
+    DH_CHECK_LAUNCH();
+    return 0;
+}

@@ -125,6 +125,24 @@ int dh_trsm_f32(dh_handle_t h, char side, char uplo, char transa, char diag, int
 int dh_potrf_f64(dh_handle_t h, char uplo, int n, double* A, int lda, int bs);
 int dh_potrf_f32(dh_handle_t h, char uplo, int n, float* A, int lda, int bs);
 
+/* Solve A X = B (A n x n SPD, upper triangle used; B n x nrhs): A is overwritten
+ * by U (dh_potrf_f64 with block bs), B by X through U^T Z = B (the L,U,T solve
+ * of triangular_solve, blas_like.h:95-128) and U X = Z (L,U,N).  LAPACK posv
+ * contract; returns 0 or the potrf info.  Synchronises. */
+int dh_posv_f64(dh_handle_t h, char uplo, int n, int nrhs, double* A, int lda, double* B, int ldb,
+                int bs);
+
+/* Mixed-precision A X = B: FP32 Cholesky (block bs) of a rounded copy of A,
+ * FP64 iterative refinement with residuals R = B - A X computed from the
+ * upper triangle of A in FP64, until the normwise backward error
+ * max_j ||r_j||_2 / (||A||_F ||x_j||_2 + ||b_j||_2) <= tol or max_iter
+ * corrections.  A and B are not modified.  *iters = corrections applied
+ * (-1: the FP32 factorisation broke down and the FP64 path was used, as in
+ * LAPACK dsposv); *berr = final backward error.  Synchronises. */
+int dh_posv_mixed(dh_handle_t h, char uplo, int n, int nrhs, const double* A, int lda,
+                  const double* B, int ldb, double* X, int ldx, int bs, double tol, int max_iter,
+                  int* iters, double* berr);
+
 /* ------------------------------------------------------- range finder */
 
 /* CholeskyQR2 in FP64: Y (m x n, m >= n) is overwritten by Q with Q^T Q = I, and

@@ -257,24 +257,28 @@ def test_cholqr2_golden(dev):
     assert rel(dh.from_dev(R), G["qr_R"]) < 1e-12
 
 
-def test_cholqr2_tall(dev):
+@pytest.mark.parametrize("m,n", [(20000, 138), (6000, 544), (4000, 600)])
+def test_cholqr2_tall(dev, m, n):
+    # n <= 544: one-launch Cholesky + inverse per sweep; n = 600: blocked fallback.
     import torch
     rng = np.random.default_rng(16)
-    Y = rng.standard_normal((20000, 138)) @ np.diag(np.logspace(0, -4, 138))
+    Y = rng.standard_normal((m, n)) @ np.diag(np.logspace(0, -4, n))
     Yd = dev_mat(Y)
-    R = empty_mat(138, 138, torch.float64)
+    R = empty_mat(n, n, torch.float64)
     assert dev.cholqr2(Yd, R) == 0
     Q, Rh = dh.from_dev(Yd), dh.from_dev(R)
-    assert np.linalg.norm(Q.T @ Q - np.eye(138)) < 1e-13
+    assert np.linalg.norm(Q.T @ Q - np.eye(n)) < 1e-13
     assert rel(Q @ Rh, Y) < 1e-13
+    np.testing.assert_array_equal(np.tril(Rh, -1), 0)
     _, Ro, info = orc.cholqr2(Y)
     assert info == 0 and rel(Rh, Ro) < 1e-10
 
 
-def test_cholqr2_breakdown(dev):
+@pytest.mark.parametrize("n", [10, 600])
+def test_cholqr2_breakdown(dev, n):
     # Exactly dependent columns: every CholeskyQR variant must report breakdown.
     rng = np.random.default_rng(17)
-    Y = rng.standard_normal((500, 10))
+    Y = rng.standard_normal((2000, n))
     Y[:, 7] = Y[:, 2]
     assert dev.cholqr2(dev_mat(Y)) > 0
 
@@ -356,3 +360,65 @@ def test_lra_f64_full_size_properties(dev):
     # Q orthonormal: ||K - QQ^T K||^2 = ||K||^2 - ||Q^T K||^2
     assert abs(e2 - (k2 - w2)) <= 1e-9 * k2
     assert 0 <= np.sqrt(e2 / k2) < 0.5
+
+
+# ------------------------------------------------------------- core solve
+
+def _rbf_core(n, d, seed=21, shift=1e-3):
+    X = dh.fill_mixture(seed, d, n, 16, 0.25)
+    return orc.rbf_tile(X, X, 0.5 / d) + shift * np.eye(n)
+
+
+@pytest.mark.parametrize("n,nrhs,bs", [(700, 13, 256), (1000, 64, 128)])
+def test_posv_f64_vs_oracle(dev, n, nrhs, bs):
+    rng = np.random.default_rng(n)
+    M = rng.standard_normal((n, n))
+    A = M.T @ M / n + np.eye(n)
+    B = rng.standard_normal((n, nrhs))
+    Ad, Bd = dev_mat(A), dev_mat(B)
+    assert dev.posv(Ad, Bd, bs) == 0
+    U, info = orc.potrf_upper(A, bs)
+    Xo = orc.trsm("L", "N", np.triu(U), orc.trsm("L", "T", np.triu(U), B))
+    assert info == 0 and rel(dh.from_dev(Bd), Xo) < 1e-10
+    assert rel(np.triu(dh.from_dev(Ad)), np.triu(U)) < 1e-12
+
+
+def test_posv_f64_reports_failure(dev):
+    A = np.eye(50)
+    A[30, 30] = -1.0
+    assert dev.posv(dev_mat(A), dev_mat(np.ones((50, 2))), 16) == 31
+
+
+def test_posv_mixed_rbf_core(dev):
+    """FP32 Cholesky + FP64 refinement on an RBF core + 1e-3 I (kappa ~ 1e6)."""
+    import torch
+    n, nrhs = 2048, 16
+    A = _rbf_core(n, 32)
+    rng = np.random.default_rng(22)
+    B = rng.standard_normal((n, nrhs))
+    X = torch.zeros((nrhs, n), dtype=torch.float64, device="cuda")
+    info, iters, berr = dev.posv_mixed(dev_mat(np.triu(A)), dev_mat(B), X, 256, 1e-10, 100)
+    assert info == 0 and berr <= 1e-10, (iters, berr)
+    Xh = dh.from_dev(X)
+    R = B - A @ Xh
+    anorm = np.linalg.norm(A)
+    be = max(np.linalg.norm(R[:, j]) / (anorm * np.linalg.norm(Xh[:, j]) + np.linalg.norm(B[:, j]))
+             for j in range(nrhs))
+    assert be <= 1e-10
+    Xref = np.linalg.solve(A, B)
+    assert rel(Xh, Xref) <= 1e-10 * np.linalg.cond(A)
+
+
+def test_posv_mixed_falls_back_to_fp64(dev):
+    # Eigenvalues down to 1e-10 relative: FP32 Cholesky breaks down, FP64 does not.
+    import torch
+    rng = np.random.default_rng(23)
+    n = 300
+    Qm, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = (Qm * np.logspace(0, -10, n)) @ Qm.T
+    A = (A + A.T) / 2
+    B = rng.standard_normal((n, 3))
+    X = torch.zeros((3, n), dtype=torch.float64, device="cuda")
+    info, iters, berr = dev.posv_mixed(dev_mat(A), dev_mat(B), X, 64, 1e-10, 30)
+    assert info == 0 and iters == -1
+    assert rel(A @ dh.from_dev(X), B) < 1e-5
