@@ -20,6 +20,11 @@ SIGNATURES: dict[str, tuple] = {
     "dh_synchronize": (_i, [_p]),
     "dh_last_error_string": (C.c_char_p, []),
     "dh_workspace_bytes": (_sz, [_p]),
+    "dh_malloc": (_i, [_p, _sz, C.POINTER(_p)]),
+    "dh_free": (_i, [_p, _p]),
+    "dh_copy_2d": (_i, [_p, _i, _i, _i, _sz, _p, _i, _p, _i]),
+    "dh_convert_f32_f64": (_i, [_p, _i, _i, _p, _i, _p, _i]),
+    "dh_convert_f64_f32": (_i, [_p, _i, _i, _p, _i, _p, _i]),
     "dh_sqnorms_f64": (_i, [_p, _i, _i, _p, _i, _p]),
     "dh_rbf_tile_f64": (_i, [_p, _i, _i, _i, _p, _i, _p, _i, _p, _p, _d, _p, _i]),
     "dh_rbf_tile_f32": (_i, [_p, _i, _i, _i, _p, _i, _p, _i, _p, _p, _f, _p, _i]),

@@ -63,6 +63,28 @@ def build_lib(jobs: int | None = None) -> Path:
     return LIB
 
 
+CPP_TESTS = ROOT / "tests" / "cpp"
+CPP_BIN = CPP_TESTS / "bin"
+
+
+def build_cpp_drivers() -> list[Path]:
+    """C++ programs written against the DArray header mirror (include/darray_hip/),
+    linked to libdarray_hip.so with an $ORIGIN-relative rpath."""
+    CPP_BIN.mkdir(exist_ok=True)
+    cxx = shutil.which("g++") or "c++"
+    rpath = "-Wl,-rpath,$ORIGIN/../../../darray_optimizer_amd"
+    out = []
+    for src in sorted(CPP_TESTS.glob("*.cpp")):
+        exe = CPP_BIN / src.stem
+        cmd = [cxx, "-std=c++17", "-O2", "-Wall", f"-I{ROOT / 'include'}", str(src), "-o", str(exe),
+               f"-L{PKG}", "-ldarray_hip", rpath]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"C++ driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        out.append(exe)
+    return out
+
+
 def build_oracle() -> Path:
     r = subprocess.run(["make", "-C", str(ORACLE), "-B", "liboracle.so"], capture_output=True,
                        text=True)
@@ -78,6 +100,7 @@ def main(argv=None) -> int:
     a = ap.parse_args(argv)
     print("built", build_lib(a.jobs), flush=True)
     if not a.lib_only:
+        print("built", *build_cpp_drivers(), flush=True)
         print("built", build_oracle(), flush=True)
     return 0
 

@@ -102,3 +102,90 @@ extern "C" size_t dh_workspace_bytes(dh_handle_t h) {
     for (int i = 0; i < DH_WS_COUNT; ++i) s += h->ws_size[i];
     return s;
 }
+
+// ------------------------------------------------------ device memory helpers
+// For callers that include no HIP header (the DArray C++ mirror): device
+// allocations on the handle's device and 2-D copies / conversions on its stream.
+
+extern "C" int dh_malloc(dh_handle_t h, size_t bytes, void** ptr) {
+    if (!h) return dh_fail(-1, "dh_malloc: NULL handle");
+    if (!ptr) return dh_fail(-3, "dh_malloc: NULL output pointer");
+    *ptr = nullptr;
+    if (bytes == 0) return 0;
+    DH_CHECK_HIP(hipSetDevice(h->device));
+    if (hipMalloc(ptr, bytes) != hipSuccess) {
+        *ptr = nullptr;
+        (void)hipGetLastError();
+        return dh_fail(DH_ERR_ALLOC, "dh_malloc: %zu bytes", bytes);
+    }
+    return 0;
+}
+
+extern "C" int dh_free(dh_handle_t h, void* ptr) {
+    if (!h) return dh_fail(-1, "dh_free: NULL handle");
+    if (!ptr) return 0;
+    DH_CHECK_HIP(hipSetDevice(h->device));
+    DH_CHECK_HIP(hipStreamSynchronize(h->stream));
+    DH_CHECK_HIP(hipFree(ptr));
+    return 0;
+}
+
+extern "C" int dh_copy_2d(dh_handle_t h, int dir, int m, int n, size_t elem, const void* src,
+                          int lds, void* dst, int ldd) {
+    const char* name = "dh_copy_2d";
+    if (!h) return dh_fail(-1, "%s: NULL handle", name);
+    if (dir != DH_COPY_H2D && dir != DH_COPY_D2H && dir != DH_COPY_D2D)
+        return dh_fail(-2, "%s: bad direction %d", name, dir);
+    if (m < 0) return dh_fail(-3, "%s: m < 0", name);
+    if (n < 0) return dh_fail(-4, "%s: n < 0", name);
+    if (elem == 0) return dh_fail(-5, "%s: elem = 0", name);
+    if (lds < max(1, m)) return dh_fail(-7, "%s: lds < m", name);
+    if (ldd < max(1, m)) return dh_fail(-9, "%s: ldd < m", name);
+    if (m == 0 || n == 0) return 0;
+    hipMemcpyKind kind = dir == DH_COPY_H2D   ? hipMemcpyHostToDevice
+                         : dir == DH_COPY_D2H ? hipMemcpyDeviceToHost
+                                              : hipMemcpyDeviceToDevice;
+    DH_CHECK_HIP(hipSetDevice(h->device));
+    DH_CHECK_HIP(hipMemcpy2DAsync(dst, elem * (size_t)ldd, src, elem * (size_t)lds,
+                                  elem * (size_t)m, n, kind, h->stream));
+    DH_CHECK_HIP(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+namespace {
+template <typename S, typename D>
+__global__ __launch_bounds__(256) void convert_kernel(int m, int n, const S* __restrict__ A,
+                                                      int lda, D* __restrict__ B, int ldb) {
+    long total = (long)m * n;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+        int i = (int)(e % m), j = (int)(e / m);
+        B[i + (size_t)j * ldb] = (D)A[i + (size_t)j * lda];
+    }
+}
+
+template <typename S, typename D>
+int convert(dh_handle_t h, const char* name, int m, int n, const S* A, int lda, D* B, int ldb) {
+    if (!h) return dh_fail(-1, "%s: NULL handle", name);
+    if (m < 0) return dh_fail(-2, "%s: m < 0", name);
+    if (n < 0) return dh_fail(-3, "%s: n < 0", name);
+    if (lda < max(1, m)) return dh_fail(-5, "%s: lda < m", name);
+    if (ldb < max(1, m)) return dh_fail(-7, "%s: ldb < m", name);
+    if (m == 0 || n == 0) return 0;
+    long total = (long)m * n;
+    long want = (total + 255) / 256, cap = 8L * h->num_cu;
+    int blocks = (int)(want < cap ? want : cap);
+    convert_kernel<S, D><<<blocks, 256, 0, h->stream>>>(m, n, A, lda, B, ldb);
+    DH_CHECK_LAUNCH();
+    return 0;
+}
+}  // namespace
+
+extern "C" int dh_convert_f32_f64(dh_handle_t h, int m, int n, const float* A, int lda, double* B,
+                                  int ldb) {
+    return convert(h, "dh_convert_f32_f64", m, n, A, lda, B, ldb);
+}
+
+extern "C" int dh_convert_f64_f32(dh_handle_t h, int m, int n, const double* A, int lda, float* B,
+                                  int ldb) {
+    return convert(h, "dh_convert_f64_f32", m, n, A, lda, B, ldb);
+}

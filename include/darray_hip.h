@@ -51,6 +51,27 @@ const char* dh_last_error_string(void);
 /* Bytes of device workspace currently held by the handle. */
 size_t dh_workspace_bytes(dh_handle_t h);
 
+/* ------------------------------------------------- device memory helpers */
+/* For hosts that include no HIP header (the DArray C++ mirror in darray_hip/).
+ * The reference allocates with cudaMallocManaged per call (dkernel.cu:135-147)
+ * and copies element by element through host bounce buffers (dkernel.cu:80-90). */
+
+/* Device allocation of `bytes` on the handle's device (*ptr = NULL for 0 bytes). */
+int dh_malloc(dh_handle_t h, size_t bytes, void** ptr);
+/* Free a dh_malloc allocation after the handle's stream has drained. */
+int dh_free(dh_handle_t h, void* ptr);
+
+#define DH_COPY_H2D 0
+#define DH_COPY_D2H 1
+#define DH_COPY_D2D 2
+/* Copy an m x n column-major block of `elem`-byte elements, src (ld lds) to
+ * dst (ld ldd), direction DH_COPY_*.  Synchronous. */
+int dh_copy_2d(dh_handle_t h, int dir, int m, int n, size_t elem, const void* src, int lds,
+               void* dst, int ldd);
+/* Device conversion B = A of an m x n block, FP32 <-> FP64. */
+int dh_convert_f32_f64(dh_handle_t h, int m, int n, const float* A, int lda, double* B, int ldb);
+int dh_convert_f64_f32(dh_handle_t h, int m, int n, const double* A, int lda, float* B, int ldb);
+
 /* ------------------------------------------------------------- RBF kernel */
 
 /* Squared norms nrm[i] = sum_f X[f + i*ldx]^2 (vecnorm, dkernel.cu:102-114). */
