@@ -5,6 +5,9 @@ Gaussian-mixture workload (16 centres in [-1,1]^128, sigma = 0.25, gamma = 0.5/d
 Y = K*Omega, CholeskyQR2, two power iterations, W = K*Q (B = W^T), inputs
 resident in HBM.  GFLOP/s counts F_MFMA + F_orth:
     F_MFMA = (q+2) * (2 n^2 d + 2 n^2 l),  F_orth = (q+1) * (6 n l^2 + 2/3 l^3)
+Besides the headline (BF16 K-passes) the line carries the FP64 solve, the
+FP16 K-passes, the 16384 core solve, the batched workload (96 problems,
+one graph) and the CPU baseline.
 Prints ONE JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
 """
 from __future__ import annotations
@@ -84,6 +87,57 @@ def core_solve(h, Xd, gamma, nc, nrhs, stream):
     return out
 
 
+def batched_workload(h, count, seed, with_ref):
+    """96 independent problems: n ~ U{2048..32768}, d in {16, 64, 256},
+    k in {32, 64, 128, 256}, p = 10, q = 1, 4-centre mixtures (sigma 0.3)
+    with +-1 labels by nearest-centre parity, gamma = 1/d; Omega ~ N(0,1)."""
+    import torch
+    import darray_optimizer_amd as dh
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    probs, flops = [], 0.0
+    for i in range(count):
+        n = int(rng.integers(2048, 32769))
+        d = int(rng.choice([16, 64, 256]))
+        k = int(rng.choice([32, 64, 128, 256]))
+        l, q = k + 10, 1
+        X, y = dh.fill_mixture(seed * 1000 + i, d, n, 4, 0.3, labels=True)
+        probs.append({"X": torch.from_numpy(X).cuda(), "y": torch.from_numpy(y).cuda(),
+                      "Omega": torch.randn((l, n), dtype=torch.float64, device="cuda",
+                                           generator=gen),
+                      "gamma": 1.0 / d, "q": q,
+                      "Q": torch.empty((l, n), dtype=torch.float64, device="cuda"),
+                      "W": torch.empty((l, n), dtype=torch.float64, device="cuda")})
+        fm, fo = lra_flops(n, d, l, q)
+        flops += fm + fo
+    r, st = h.lra_batched(probs, prec=dh.DH_PREC_BF16)  # plan, autotuning, graph capture
+    assert r == 0, st
+    torch.cuda.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        h.lra_batched(probs, prec=dh.DH_PREC_BF16)
+    torch.cuda.synchronize()
+    s = (time.perf_counter() - t0) / reps
+    out = {"problems": count, "p": 10, "q": 1, "s_per_batch": s, "gflops": flops / s / 1e9,
+           "n_range": [min(p["X"].shape[0] for p in probs), max(p["X"].shape[0] for p in probs)]}
+    if with_ref:
+        # Error ratio vs FP64 on every 8th problem (FP64 solve + FP64 error pass each).
+        ratios = []
+        for p in probs[::8]:
+            e2, k2 = h.lra_error(p["X"], p["gamma"], p["Q"], p["W"], y=p["y"])
+            Q64, W64 = torch.empty_like(p["Q"]), torch.empty_like(p["W"])
+            h.lra(p["X"], p["Omega"], p["gamma"], p["q"], Q64, W64, y=p["y"], prec=dh.DH_PREC_F64)
+            f2, _ = h.lra_error(p["X"], p["gamma"], Q64, W64, y=p["y"])
+            ratios.append(float(np.sqrt(max(e2, 0.0) / max(f2, 1e-300))))
+        out["err_ratio_vs_fp64"] = {"checked": len(ratios), "max": max(ratios),
+                                    "mean": float(np.mean(ratios))}
+    del probs
+    torch.cuda.empty_cache()
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +154,7 @@ def main(argv=None) -> int:
     ap.add_argument("--core", type=int, default=16384, help="core order (0: skip)")
     ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--batched", type=int, default=96, help="batched problems (0: skip)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
@@ -137,6 +192,7 @@ def main(argv=None) -> int:
 
     res = {}
     for name, prec, steps, warm in (("fp64", dh.DH_PREC_F64, a.steps_fp64, min(a.warmup, 1)),
+                                    ("f16", dh.DH_PREC_F16, a.steps, a.warmup),
                                     ("mixed", dh.DH_PREC_BF16, a.steps, a.warmup)):
         s = timed(prec, steps, warm)
         r = {"s_per_solve": s, "gflops": (f_mfma + f_orth) / s / 1e9, "steps": steps}
@@ -144,7 +200,7 @@ def main(argv=None) -> int:
             e2, k2 = h.lra_error(Xd, gamma, Q, W)
             r["rel_err"] = float(np.sqrt(max(e2, 0.0) / k2))
         res[name] = r
-    mixed, fp64 = res["mixed"], res["fp64"]
+    mixed, fp64, f16 = res["mixed"], res["fp64"], res["f16"]
 
     # Dominant kernel of each path: one fused K*Omega pass, timed with events
     # on the stream it runs on (the handle runs on torch's current stream).
@@ -180,12 +236,16 @@ def main(argv=None) -> int:
         "config": {"workload": "rbf_lra_gaussian_mixture", "n": n, "d": d, "k": a.k, "p": a.p,
                    "q": q, "l": l, "gamma": gamma, "centres": 16, "sigma": 0.25,
                    "precision": "bf16 K-passes (fp32 accumulate), fp64 CholeskyQR2"},
-        "mixed": mixed, "fp64": fp64,
+        "mixed": mixed, "fp64": fp64, "f16": f16,
         "speedup_vs_fp64": fp64["s_per_solve"] / mixed["s_per_solve"],
         "roofline": roofline,
     }
     if "rel_err" in mixed:
         line["err_ratio_vs_fp64"] = mixed["rel_err"] / fp64["rel_err"]
+        f16["err_ratio_vs_fp64"] = f16["rel_err"] / fp64["rel_err"]
+    f16["speedup_vs_fp64"] = fp64["s_per_solve"] / f16["s_per_solve"]
+    if a.batched > 0:
+        line["batched"] = batched_workload(h, a.batched, 3, not a.no_error)
     if a.core > 0:
         line["core_solve"] = core_solve(h, Xd, gamma, a.core, a.nrhs, stream)
     if not a.no_cpu:

@@ -14,9 +14,9 @@ import ctypes as C
 import numpy as np
 
 from . import _capi
-from ._capi import DH_PREC_BF16, DH_PREC_F64, DHError, call, lib
+from ._capi import DH_PREC_BF16, DH_PREC_F16, DH_PREC_F64, DHError, call, lib
 
-__all__ = ["Handle", "DHError", "DH_PREC_F64", "DH_PREC_BF16", "lib", "fill_uniform",
+__all__ = ["Handle", "DHError", "DH_PREC_F64", "DH_PREC_BF16", "DH_PREC_F16", "lib", "fill_uniform",
            "fill_normal", "fill_mixture", "mt19937_raw", "to_dev", "from_dev"]
 
 
@@ -113,6 +113,15 @@ class Handle:
              _ptr(yi), _ptr(yj), float(gamma), _ptr(O), _ld(O), _ptr(Y), _ld(Y))
         return Y
 
+    def rbf_project_f16(self, Xi, Xj, O, gamma, Y, yi=None, yj=None):
+        """As rbf_project_bf16 with FP16 operands (FP64 in/out)."""
+        m, d = Xi.shape
+        n = Xj.shape[0]
+        l = O.shape[0]
+        call("dh_rbf_project_f16", self.h, m, n, d, l, _ptr(Xi), _ld(Xi), _ptr(Xj), _ld(Xj),
+             _ptr(yi), _ptr(yj), float(gamma), _ptr(O), _ld(O), _ptr(Y), _ld(Y))
+        return Y
+
     # -------------------------------------------------------------- dense
     def gemm(self, ta, tb, A, B, C_, alpha=1.0, beta=0.0):
         import torch
@@ -172,6 +181,21 @@ class Handle:
         l = Omega.shape[0]
         return call("dh_lra", self.h, prec, n, d, l, q, _ptr(X), _ld(X), _ptr(y), float(gamma),
                     _ptr(Omega), _ld(Omega), _ptr(Q), _ld(Q), _ptr(W), _ld(W))
+
+    def lra_batched(self, problems, prec=DH_PREC_BF16):
+        """dh_lra_batched over a list of dicts with keys X, Omega, gamma, q, Q, W
+        and optionally y (tensors as for lra).  Returns (status, per-problem statuses)."""
+        arr = (_capi.LraProblem * len(problems))()
+        for i, p in enumerate(problems):
+            X, Om, Q, W = p["X"], p["Omega"], p["Q"], p["W"]
+            y = p.get("y")
+            arr[i] = _capi.LraProblem(X.shape[0], X.shape[1], Om.shape[0], p["q"], X.data_ptr(),
+                                      _ld(X), None if y is None else y.data_ptr(),
+                                      float(p["gamma"]), Om.data_ptr(), _ld(Om), Q.data_ptr(),
+                                      _ld(Q), W.data_ptr(), _ld(W))
+        st = (C.c_int * max(1, len(problems)))()
+        r = call("dh_lra_batched", self.h, prec, len(problems), arr, st)
+        return r, list(st)[:len(problems)]
 
     def lra_error(self, X, gamma, Q, W, y=None):
         """(||K - Q W^T||_F^2, ||K||_F^2) in FP64."""

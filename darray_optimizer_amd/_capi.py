@@ -13,6 +13,14 @@ LIB_PATH = Path(__file__).resolve().with_name("libdarray_hip.so")
 _i, _c, _d, _f, _p = C.c_int, C.c_char, C.c_double, C.c_float, C.c_void_p
 _u32, _u64, _sz = C.c_uint32, C.c_uint64, C.c_size_t
 
+
+class LraProblem(C.Structure):
+    """dh_lra_problem of include/darray_hip.h."""
+    _fields_ = [("n", _i), ("d", _i), ("l", _i), ("q", _i), ("X", _p), ("ldx", _i), ("y", _p),
+                ("gamma", _d), ("Omega", _p), ("ldo", _i), ("Q", _p), ("ldq", _i), ("W", _p),
+                ("ldw", _i)]
+
+
 # name -> (restype, argtypes); mirrors include/darray_hip.h one to one.
 SIGNATURES: dict[str, tuple] = {
     "dh_create": (_i, [_i, _p, C.POINTER(_p)]),
@@ -32,6 +40,8 @@ SIGNATURES: dict[str, tuple] = {
                                 _i]),
     "dh_rbf_project_bf16": (_i, [_p, _i, _i, _i, _i, _p, _i, _p, _i, _p, _p, _d, _p, _i, _p,
                                  _i]),
+    "dh_rbf_project_f16": (_i, [_p, _i, _i, _i, _i, _p, _i, _p, _i, _p, _p, _d, _p, _i, _p,
+                                _i]),
     "dh_gemm_f64": (_i, [_p, _c, _c, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p, _i]),
     "dh_gemm_f32": (_i, [_p, _c, _c, _i, _i, _i, _f, _p, _i, _p, _i, _f, _p, _i]),
     "dh_syrk_ut_f64": (_i, [_p, _i, _i, _d, _p, _i, _d, _p, _i]),
@@ -45,6 +55,7 @@ SIGNATURES: dict[str, tuple] = {
                            C.POINTER(_d)]),
     "dh_cholqr2_f64": (_i, [_p, _i, _i, _p, _i, _p, _i]),
     "dh_lra": (_i, [_p, _i, _i, _i, _i, _i, _p, _i, _p, _d, _p, _i, _p, _i, _p, _i]),
+    "dh_lra_batched": (_i, [_p, _i, _i, C.POINTER(LraProblem), C.POINTER(_i)]),
     "dh_lra_error": (_i, [_p, _i, _i, _i, _p, _i, _p, _d, _p, _i, _p, _i, C.POINTER(_d)]),
     "dh_mt19937_raw": (_i, [_u32, _i, C.POINTER(_u32)]),
     "dh_fill_uniform": (_i, [_u32, _d, _d, _i, _i, C.POINTER(_d), _i]),
@@ -55,6 +66,7 @@ SIGNATURES: dict[str, tuple] = {
 
 DH_PREC_F64 = 0
 DH_PREC_BF16 = 1
+DH_PREC_F16 = 2
 DH_ERR_HIP = -1000
 DH_ERR_ALLOC = -1001
 DH_ERR_UNSUPPORTED = -1002

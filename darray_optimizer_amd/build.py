@@ -33,7 +33,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall",
           "-Wno-unused-parameter", "-Wno-unused-function"]
 
 SOURCES = ["handle.hip", "dense.hip", "small_spd.hip", "solve.hip", "rbf_f64.hip", "rbf_bf16.hip",
-           "lra.hip", "gen.cpp", "shim.cpp"]
+           "lra.hip", "batch.hip", "gen.cpp", "shim.cpp"]
 
 
 def _compile(src: str) -> Path:

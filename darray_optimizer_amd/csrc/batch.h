@@ -1,0 +1,68 @@
+// Device-side tables of the batched range finder (batch.hip) and the batched
+// BF16 kernels that read them (rbf_bf16.hip).
+#pragma once
+
+#include "dh_common.h"
+
+// One point set prepared for BF16 passes.
+struct DhBf16Set {
+    int n, d, dpad, npad;      // points, features, padded features, rows padded to 32
+    const double* X;           // d x n, ld ldx
+    int ldx;
+    double neg_gl2e;           // -gamma log2(e)
+    double* part;              // ceil(n/256) x d column partial sums
+    double* mean;              // d
+    void* xb;                  // npad x dpad BF16 (centred points)
+    float* a;                  // npad: -gamma log2(e) |xb_i|^2
+};
+
+// Packing of one projection operand: src (n x l, ld lds; may be NULL for
+// none), labels y (may be NULL) -> Op, ncb_pack 16-column blocks wide.
+struct DhPackJob {
+    int n, l, nchunks, ncb_pack;
+    const double* src;
+    int lds;
+    const double* y;
+    void* Op;
+};
+
+// One fused projection Y (m x l) = K(X, X) * Op of a prepared set.
+struct DhProjJob {
+    int m, nchunks, l, ncb_pack;
+    const void* xb;
+    const float* a;
+    float c2, inv_c2;
+    const void* Op;
+    const double* y;
+    double* Y;
+    int ldy;
+};
+
+// A work item of a persistent projection kernel: row block, column group of a job.
+struct DhProjItem {
+    int job, rblk, cg, pad;
+};
+
+// Tile class of the BF16 projection kernel.
+struct DhBf16Class {
+    int dk;     // 16-byte feature groups / 4 (dpad / 32)
+    int ncb;    // 16-column blocks per workgroup
+    int waves;  // 8 or 4
+    int pp;     // ping-pong schedule
+};
+
+int dh_bf16_dpad(int d);
+// Waves per workgroup the BF16 projection uses for dpad.
+int dh_bf16_waves(int dpad);
+// Column blocking of an l-wide pass: groups of ncb blocks, Op packed ncb_pack wide.
+void dh_bf16_columns(int l, int* ncb, int* groups, int* ncb_pack);
+size_t dh_bf16_packed_bytes(int n, int ncb_pack);
+
+// Batched launches over device tables (count entries each).
+int dh_bf16_prepare_batched(dh_handle_t h, const DhBf16Set* sets, int count, int max_n, int max_d,
+                            int max_npad);
+int dh_bf16_pack_batched(dh_handle_t h, const DhPackJob* jobs, int count, long max_threads);
+// Persistent projection of class c over nitems items, work handed out by
+// atomically incrementing *counter (zero on entry); grid sized to fill the chip.
+int dh_bf16_project_persistent(dh_handle_t h, DhBf16Class c, const DhProjJob* jobs,
+                               const DhProjItem* items, int nitems, int* counter);

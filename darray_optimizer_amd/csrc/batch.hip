@@ -1,0 +1,501 @@
+// Batched randomised range finder (dh_lra_batched) and the autotuner table.
+//
+// A batch of independent problems (n_i, d_i, l_i, shared q) runs stage by
+// stage, each stage one launch over a device table of all problems:
+//   prepare  : column means, BF16 centred points, squared norms
+//   per pass : pack the operand (Omega or Q) -> persistent BF16 projection
+//              kernels, one per tile class (dpad, column blocks), pulling
+//              (problem, row block) items from an atomic queue, longest
+//              problem first -> CholeskyQR2 (batched split-K Gram, one
+//              workgroup Cholesky per problem, row-parallel solve), twice
+//   copies   : W -> Q after each power iteration
+// and the sequence is captured once in a hipGraph and replayed while the
+// problem table is unchanged.  Queue counters are cleared by a fill kernel
+// inside the graph (no memset nodes).
+//
+// Breakdown: a problem whose Gram matrix is numerically singular keeps its
+// flag in the device info words; after the graph the host redoes such
+// problems alone with dh_lra, which falls back to shifted CholeskyQR3.
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "batch.h"
+#include "mfma_tile.h"
+
+namespace {
+
+constexpr float kLog2e = 1.4426950408889634f;
+constexpr int kGramK = 4096;  // rows of Y per split-K Gram workgroup
+
+// ------------------------------------------------------------------ kernels
+
+struct GramJob {
+    int m, n, ldy, nsplit;
+    const double* Y;
+    double* part;  // nsplit x (n x n)
+    double* G;     // n x n, ld n
+};
+
+// Upper 64 x 64 tiles of Y^T Y over a k-range: blockIdx.x = upper tile,
+// blockIdx.y = split, blockIdx.z = job.
+__global__ __launch_bounds__(256) void gram_batched_kernel(const GramJob* __restrict__ jobs) {
+    const GramJob& j = jobs[blockIdx.z];
+    const int nt = (j.n + 63) / 64;
+    if ((int)blockIdx.x >= nt * (nt + 1) / 2 || (int)blockIdx.y >= j.nsplit) return;
+    int bi = 0, rem = blockIdx.x;
+    while (rem >= nt - bi) {
+        rem -= nt - bi;
+        ++bi;
+    }
+    const int bj = bi + rem;
+    __shared__ double sA[kTileBK][64 + kTilePad], sB[kTileBK][64 + kTilePad];
+    dh_d4 acc[2][2];
+    zero_acc<double, 2, 2>(acc);
+    const int kbeg = blockIdx.y * kGramK, kend = min(j.m, kbeg + kGramK);
+    tile_mma<double, 64, 64, true, false>(acc, sA, sB, j.n, j.n, 64 * bi, 64 * bj, kbeg, kend, j.Y,
+                                          j.ldy, j.Y, j.ldy);
+    double* P = j.part + (size_t)blockIdx.y * j.n * j.n;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int i = 64 * bi + tile_row<double, 64, 64>(a, r);
+                int c = 64 * bj + tile_col<64, 64>(b);
+                if (i < j.n && c < j.n) P[i + (size_t)c * j.n] = acc[a][b][r];
+            }
+}
+
+// G = sum of the split partials on the upper triangle (fixed order: deterministic).
+__global__ __launch_bounds__(256) void gram_reduce_batched_kernel(const GramJob* __restrict__ jobs) {
+    const GramJob& j = jobs[blockIdx.y];
+    const size_t nn = (size_t)j.n * j.n;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < nn; e += (size_t)gridDim.x * 256) {
+        int i = (int)(e % j.n), c = (int)(e / j.n);
+        if (i > c) continue;
+        double s = 0.0;
+        for (int t = 0; t < j.nsplit; ++t) s += j.part[t * nn + e];
+        j.G[e] = s;
+    }
+}
+
+struct CopyJob {
+    int m, n, lds, ldd;
+    const double* src;
+    double* dst;
+};
+
+__global__ __launch_bounds__(256) void copy_batched_kernel(const CopyJob* __restrict__ jobs) {
+    const CopyJob& j = jobs[blockIdx.y];
+    const long total = (long)j.m * j.n;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+        int i = (int)(e % j.m), c = (int)(e / j.m);
+        j.dst[i + (size_t)c * j.ldd] = j.src[i + (size_t)c * j.lds];
+    }
+}
+
+__global__ __launch_bounds__(256) void fill_int_kernel(int* __restrict__ p, int count, int value) {
+    for (int e = blockIdx.x * 256 + threadIdx.x; e < count; e += gridDim.x * 256) p[e] = value;
+}
+
+// --------------------------------------------------------------- the plan
+
+struct ClassRange {
+    DhBf16Class cls;
+    int first, count;  // items[first, first + count)
+};
+
+struct Plan {
+    int prec = -1, q = 0;
+    std::vector<dh_lra_problem> probs;
+    void* mem = nullptr;
+    size_t bytes = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int* infos = nullptr;   // 2 per problem: status of the latest Q / W CholeskyQR step
+    int* sticky = nullptr;  // 1 per problem: some CholeskyQR step broke down
+
+    ~Plan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (mem) (void)hipFree(mem);
+    }
+};
+
+struct TuneTable {
+    std::map<uint64_t, int> choice;
+};
+
+// Bump allocator over one device block (256-byte aligned pieces).
+struct Carve {
+    size_t off = 0;
+    size_t take(size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) & ~size_t(255);
+        return o;
+    }
+};
+
+bool same_problems(const Plan& p, int prec, int count, const dh_lra_problem* probs) {
+    return p.exec && p.prec == prec && (int)p.probs.size() == count &&
+           std::memcmp(p.probs.data(), probs, sizeof(dh_lra_problem) * count) == 0;
+}
+
+}  // namespace
+
+uint64_t dh_tune_key_bf16(int dk, int ncb, int waves, int rows) {
+    int lg = 0;
+    while ((1 << lg) < rows && lg < 30) ++lg;
+    return ((uint64_t)1 << 56) | ((uint64_t)dk << 40) | ((uint64_t)ncb << 32) |
+           ((uint64_t)waves << 24) | (uint64_t)lg;
+}
+
+// ------------------------------------------------------------ tune table
+
+bool dh_tune_enabled() {
+    static int on = -1;
+    if (on < 0) {
+        const char* e = std::getenv("DH_TUNE");
+        on = (e && e[0] == '0') ? 0 : 1;
+    }
+    return on != 0;
+}
+
+int dh_tune_lookup(dh_handle_t h, uint64_t key) {
+    if (!h->tune) return -1;
+    auto& m = static_cast<TuneTable*>(h->tune)->choice;
+    auto it = m.find(key);
+    return it == m.end() ? -1 : it->second;
+}
+
+void dh_tune_store(dh_handle_t h, uint64_t key, int choice) {
+    if (!h->tune) h->tune = new TuneTable();
+    static_cast<TuneTable*>(h->tune)->choice[key] = choice;
+}
+
+void dh_tune_release(dh_handle_t h) {
+    delete static_cast<TuneTable*>(h->tune);
+    h->tune = nullptr;
+}
+
+void dh_batch_release(dh_handle_t h) {
+    delete static_cast<Plan*>(h->batch);
+    h->batch = nullptr;
+}
+
+namespace {
+
+// Decide the ping-pong choice of each class by timing both schedules on the
+// first projection pass (identical results: only the wave schedule differs).
+// Runs eagerly, before the capture.
+int tune_classes(dh_handle_t h, std::vector<ClassRange>& classes, const DhProjJob* jobs,
+                 const DhProjItem* items, int* counters) {
+    hipEvent_t e0, e1;
+    DH_CHECK_HIP(hipEventCreate(&e0));
+    DH_CHECK_HIP(hipEventCreate(&e1));
+    int status = 0;
+    for (size_t c = 0; c < classes.size() && !status; ++c) {
+        ClassRange& cr = classes[c];
+        if (cr.cls.waves != 8) {
+            // One schedule only; run it once eagerly so that its one-time
+            // launch set-up happens outside the graph capture.
+            cr.cls.pp = 0;
+            fill_int_kernel<<<1, 256, 0, h->stream>>>(counters, 1, 0);
+            status = dh_bf16_project_persistent(h, cr.cls, jobs, items + cr.first, cr.count,
+                                                counters);
+            continue;
+        }
+        uint64_t key = dh_tune_key_bf16(cr.cls.dk, cr.cls.ncb, 8, cr.count * 256) ^ ((uint64_t)1 << 60);
+        int known = dh_tune_lookup(h, key);
+        if (known >= 0 || !dh_tune_enabled()) {
+            cr.cls.pp = known >= 0 ? known : 1;
+            continue;
+        }
+        float best = 1e30f;
+        int best_pp = 1;
+        for (int pp = 1; pp >= 0 && !status; --pp) {
+            DhBf16Class k = cr.cls;
+            k.pp = pp;
+            fill_int_kernel<<<1, 256, 0, h->stream>>>(counters, 1, 0);
+            if ((status = hipGetLastError() != hipSuccess ? DH_ERR_HIP : 0)) break;
+            (void)hipEventRecord(e0, h->stream);
+            status = dh_bf16_project_persistent(h, k, jobs, items + cr.first, cr.count, counters);
+            (void)hipEventRecord(e1, h->stream);
+            if (!status && hipEventSynchronize(e1) == hipSuccess) {
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (ms < best) {
+                    best = ms;
+                    best_pp = pp;
+                }
+            }
+        }
+        cr.cls.pp = best_pp;
+        dh_tune_store(h, key, best_pp);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return status;
+}
+
+int build_plan(dh_handle_t h, Plan& P, int prec, int count, const dh_lra_problem* probs) {
+    P.prec = prec;
+    P.q = probs[0].q;
+    P.probs.assign(probs, probs + count);
+    const int passes = P.q + 2;
+
+    // ---- sizes and layout
+    struct Inst {
+        int dpad, npad, nchunks, ncb, groups, ncb_pack, waves, rows, nsplit;
+        size_t o_part, o_mean, o_xb, o_a, o_op, o_g, o_gpart, o_dinv;
+    };
+    std::vector<Inst> in(count);
+    Carve cv;
+    const size_t o_sets = cv.take(sizeof(DhBf16Set) * count);
+    const size_t o_pack = cv.take(sizeof(DhPackJob) * count * 2);
+    const size_t o_proj = cv.take(sizeof(DhProjJob) * count * 2);
+    const size_t o_gram = cv.take(sizeof(GramJob) * count * 2);
+    const size_t o_spd = cv.take(sizeof(DhSpdJob) * count * 2);
+    const size_t o_copy = cv.take(sizeof(CopyJob) * count);
+    const size_t o_info = cv.take(sizeof(int) * count * 3);
+    int max_n = 0, max_d = 0, max_npad = 0, max_l = 0, max_tri = 0, max_split = 0;
+    long max_pack_threads = 0, max_nn = 0, max_nl = 0;
+    size_t nitems = 0;
+    for (int i = 0; i < count; ++i) {
+        const dh_lra_problem& p = probs[i];
+        Inst& s = in[i];
+        s.dpad = dh_bf16_dpad(p.d);
+        s.npad = dh_cdiv(p.n, 32) * 32;
+        s.nchunks = dh_cdiv(p.n, 32);
+        dh_bf16_columns(p.l, &s.ncb, &s.groups, &s.ncb_pack);
+        s.waves = dh_bf16_waves(s.dpad);
+        s.rows = dh_cdiv(p.n, s.waves * 32);
+        s.nsplit = dh_cdiv(p.n, kGramK);
+        s.o_part = cv.take(sizeof(double) * (size_t)dh_cdiv(p.n, 256) * p.d);
+        s.o_mean = cv.take(sizeof(double) * p.d);
+        s.o_xb = cv.take((size_t)s.npad * s.dpad * 2);
+        s.o_a = cv.take(sizeof(float) * s.npad);
+        s.o_op = cv.take(dh_bf16_packed_bytes(p.n, s.ncb_pack));
+        s.o_g = cv.take(sizeof(double) * (size_t)p.l * p.l);
+        s.o_gpart = cv.take(sizeof(double) * (size_t)s.nsplit * p.l * p.l);
+        s.o_dinv = cv.take(sizeof(double) * 256 * (size_t)dh_cdiv(p.l, 16));
+        nitems += (size_t)s.rows * s.groups;
+        max_n = std::max(max_n, p.n);
+        max_d = std::max(max_d, p.d);
+        max_npad = std::max(max_npad, s.npad);
+        max_l = std::max(max_l, p.l);
+        int nt = dh_cdiv(p.l, 64);
+        max_tri = std::max(max_tri, nt * (nt + 1) / 2);
+        max_split = std::max(max_split, s.nsplit);
+        max_pack_threads = std::max(max_pack_threads, (long)s.nchunks * s.ncb_pack * 16);
+        max_nn = std::max(max_nn, (long)p.l * p.l);
+        max_nl = std::max(max_nl, (long)p.n * p.l);
+    }
+    const size_t o_items = cv.take(sizeof(DhProjItem) * nitems);
+
+    // ---- tile classes and the longest-first work list
+    std::vector<DhProjItem> items;
+    items.reserve(nitems);
+    std::vector<ClassRange> classes;
+    {
+        std::map<std::pair<int, int>, std::vector<int>> by_class;  // (dk, ncb) -> problems
+        for (int i = 0; i < count; ++i) by_class[{in[i].dpad / 32, in[i].ncb}].push_back(i);
+        for (auto& kv : by_class) {
+            std::vector<int>& ids = kv.second;
+            std::stable_sort(ids.begin(), ids.end(),
+                             [&](int a, int b) { return probs[a].n > probs[b].n; });
+            ClassRange cr;
+            cr.cls = {kv.first.first, kv.first.second, dh_bf16_waves(kv.first.first * 32), 1};
+            cr.first = (int)items.size();
+            for (int i : ids)
+                for (int r = 0; r < in[i].rows; ++r)
+                    for (int g = 0; g < in[i].groups; ++g) items.push_back({i, r, g, 0});
+            cr.count = (int)items.size() - cr.first;
+            classes.push_back(cr);
+        }
+    }
+    const size_t o_counters = cv.take(sizeof(int) * (size_t)passes * classes.size() + 64);
+
+    P.bytes = cv.off;
+    DH_CHECK_HIP(hipMalloc(&P.mem, P.bytes));
+    char* base = (char*)P.mem;
+    auto at = [&](size_t o) { return (void*)(base + o); };
+
+    // ---- tables (host images, uploaded before capture)
+    std::vector<DhBf16Set> sets(count);
+    std::vector<DhPackJob> pack(2 * count);
+    std::vector<DhProjJob> proj(2 * count);
+    std::vector<GramJob> gram(2 * count);
+    std::vector<DhSpdJob> spd(2 * count);
+    std::vector<CopyJob> copy(count);
+    P.infos = (int*)at(o_info);
+    P.sticky = P.infos + 2 * count;
+    for (int i = 0; i < count; ++i) {
+        const dh_lra_problem& p = probs[i];
+        const Inst& s = in[i];
+        const double neg = -p.gamma * (double)kLog2e;
+        sets[i] = {p.n, p.d, s.dpad, s.npad, p.X, p.ldx, neg, (double*)at(s.o_part),
+                   (double*)at(s.o_mean), at(s.o_xb), (float*)at(s.o_a)};
+        for (int t = 0; t < 2; ++t) {
+            pack[t * count + i] = {p.n, p.l, s.nchunks, s.ncb_pack, t == 0 ? p.Omega : p.Q,
+                                   t == 0 ? p.ldo : p.ldq, p.y, at(s.o_op)};
+            const float c2 = (float)(2.0 * p.gamma) * kLog2e;
+            proj[t * count + i] = {p.n, s.nchunks, p.l, s.ncb_pack, at(s.o_xb), (float*)at(s.o_a),
+                                   c2, c2 > 0.f ? 1.f / c2 : 0.f, at(s.o_op), p.y,
+                                   t == 0 ? p.Q : p.W, t == 0 ? p.ldq : p.ldw};
+            double* Y = t == 0 ? p.Q : p.W;
+            int ldy = t == 0 ? p.ldq : p.ldw;
+            gram[t * count + i] = {p.n, p.l, ldy, s.nsplit, Y, (double*)at(s.o_gpart),
+                                   (double*)at(s.o_g)};
+            spd[t * count + i] = {p.n, p.l, ldy, p.l, Y, (double*)at(s.o_g), (double*)at(s.o_dinv),
+                                  P.infos + t * count + i, P.sticky + i};
+        }
+        copy[i] = {p.n, p.l, p.ldw, p.ldq, p.W, p.Q};
+    }
+    auto up = [&](size_t o, const void* src, size_t bytes) {
+        return hipMemcpyAsync(at(o), src, bytes, hipMemcpyHostToDevice, h->stream);
+    };
+    DH_CHECK_HIP(up(o_sets, sets.data(), sizeof(DhBf16Set) * count));
+    DH_CHECK_HIP(up(o_pack, pack.data(), sizeof(DhPackJob) * 2 * count));
+    DH_CHECK_HIP(up(o_proj, proj.data(), sizeof(DhProjJob) * 2 * count));
+    DH_CHECK_HIP(up(o_gram, gram.data(), sizeof(GramJob) * 2 * count));
+    DH_CHECK_HIP(up(o_spd, spd.data(), sizeof(DhSpdJob) * 2 * count));
+    DH_CHECK_HIP(up(o_copy, copy.data(), sizeof(CopyJob) * count));
+    DH_CHECK_HIP(up(o_items, items.data(), sizeof(DhProjItem) * items.size()));
+    DH_CHECK_HIP(hipStreamSynchronize(h->stream));
+
+    const DhBf16Set* d_sets = (const DhBf16Set*)at(o_sets);
+    const DhPackJob* d_pack = (const DhPackJob*)at(o_pack);
+    const DhProjJob* d_proj = (const DhProjJob*)at(o_proj);
+    const GramJob* d_gram = (const GramJob*)at(o_gram);
+    const DhSpdJob* d_spd = (const DhSpdJob*)at(o_spd);
+    const CopyJob* d_copy = (const CopyJob*)at(o_copy);
+    const DhProjItem* d_items = (const DhProjItem*)at(o_items);
+    int* d_counters = (int*)at(o_counters);
+    const int ncounters = passes * (int)classes.size();
+
+    // ---- eager preparation + autotuning of the projection schedules
+    DH_CHECK_STATUS(dh_bf16_prepare_batched(h, d_sets, count, max_n, max_d, max_npad));
+    DH_CHECK_STATUS(dh_bf16_pack_batched(h, d_pack, count, max_pack_threads));
+    DH_CHECK_STATUS(tune_classes(h, classes, d_proj, d_items, d_counters));
+    // Warm the batched CholeskyQR launcher (it sets a kernel attribute once).
+    DH_CHECK_STATUS(dh_cholqr_step_batched(h, d_spd, 0, max_l, max_n, false));
+    DH_CHECK_HIP(hipStreamSynchronize(h->stream));
+
+    // ---- capture
+    auto orth = [&](int t) -> int {
+        const GramJob* gj = d_gram + t * count;
+        const DhSpdJob* sj = d_spd + t * count;
+        for (int sweep = 0; sweep < 2; ++sweep) {
+            gram_batched_kernel<<<dim3(max_tri, max_split, count), 256, 0, h->stream>>>(gj);
+            DH_CHECK_LAUNCH();
+            gram_reduce_batched_kernel<<<dim3(dh_cdiv(max_nn, 256), count), 256, 0, h->stream>>>(gj);
+            DH_CHECK_LAUNCH();
+            DH_CHECK_STATUS(dh_cholqr_step_batched(h, sj, count, max_l, max_n, false));
+        }
+        return 0;
+    };
+    DH_CHECK_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int st = 0;
+    do {
+        fill_int_kernel<<<dh_cdiv(ncounters, 256), 256, 0, h->stream>>>(d_counters, ncounters, 0);
+        fill_int_kernel<<<dh_cdiv(3 * count, 256), 256, 0, h->stream>>>(P.infos, 3 * count, 0);
+        if ((st = dh_bf16_prepare_batched(h, d_sets, count, max_n, max_d, max_npad))) break;
+        for (int pass = 0; pass < passes && !st; ++pass) {
+            const int t = pass == 0 ? 0 : 1;  // operand Omega / Q, output Q / W
+            if ((st = dh_bf16_pack_batched(h, d_pack + t * count, count, max_pack_threads))) break;
+            for (size_t c = 0; c < classes.size() && !st; ++c)
+                st = dh_bf16_project_persistent(h, classes[c].cls, d_proj + t * count,
+                                                d_items + classes[c].first, classes[c].count,
+                                                d_counters + pass * (int)classes.size() + (int)c);
+            if (st || pass == passes - 1) break;
+            if ((st = orth(t))) break;
+            if (pass >= 1) {
+                copy_batched_kernel<<<dim3(std::min<long>(dh_cdiv(max_nl, 256), 4096), count), 256,
+                                      0, h->stream>>>(d_copy);
+                if (hipGetLastError() != hipSuccess) st = DH_ERR_HIP;
+            }
+        }
+    } while (0);
+    hipGraph_t graph = nullptr;
+    hipError_t ec = hipStreamEndCapture(h->stream, &graph);
+    if (st) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    if (ec != hipSuccess) return dh_fail(DH_ERR_HIP, "dh_lra_batched: capture failed: %s",
+                                         hipGetErrorString(ec));
+    P.graph = graph;
+    DH_CHECK_HIP(hipGraphInstantiate(&P.exec, graph, nullptr, nullptr, 0));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int dh_lra_batched(dh_handle_t h, int prec, int count, const dh_lra_problem* problems,
+                              int* status) {
+    const char* name = "dh_lra_batched";
+    if (!h) return dh_fail(-1, "%s: NULL handle", name);
+    if (prec != DH_PREC_F64 && prec != DH_PREC_BF16 && prec != DH_PREC_F16)
+        return dh_fail(-2, "%s: bad prec %d", name, prec);
+    if (count < 0) return dh_fail(-3, "%s: count < 0", name);
+    if (count > 0 && !problems) return dh_fail(-4, "%s: problems is NULL", name);
+    if (count == 0) return 0;
+    for (int i = 0; i < count; ++i) {
+        const dh_lra_problem& p = problems[i];
+        if (p.q != problems[0].q || p.q < 0)
+            return dh_fail(-4, "%s: problem %d: q must be >= 0 and shared (%d vs %d)", name, i, p.q,
+                           problems[0].q);
+        if (p.n < 1 || p.d < 1 || p.l < 1 || p.l > p.n || p.ldx < p.d || p.ldo < p.n ||
+            p.ldq < p.n || p.ldw < p.n || !p.X || !p.Omega || !p.Q || !p.W)
+            return dh_fail(-4, "%s: problem %d: bad sizes or pointers (n=%d d=%d l=%d)", name, i,
+                           p.n, p.d, p.l);
+        if (prec == DH_PREC_BF16 && (p.d > 256 || p.l > 272))
+            return dh_fail(-4, "%s: problem %d: BF16 batches need d <= 256 and l <= 272", name, i);
+    }
+    std::vector<int> st(count, 0);
+    int first = 0;
+    if (prec != DH_PREC_BF16) {
+        for (int i = 0; i < count; ++i) {
+            const dh_lra_problem& p = problems[i];
+            st[i] = dh_lra(h, prec, p.n, p.d, p.l, p.q, p.X, p.ldx, p.y, p.gamma, p.Omega,
+                                     p.ldo, p.Q, p.ldq, p.W, p.ldw);
+            if (st[i] < 0) return st[i];
+        }
+    } else {
+        DH_CHECK_HIP(hipSetDevice(h->device));
+        Plan* P = static_cast<Plan*>(h->batch);
+        if (!P || !same_problems(*P, prec, count, problems)) {
+            dh_batch_release(h);
+            P = new Plan();
+            h->batch = P;
+            int s = build_plan(h, *P, prec, count, problems);
+            if (s) {
+                dh_batch_release(h);
+                return s;
+            }
+        }
+        DH_CHECK_HIP(hipGraphLaunch(P->exec, h->stream));
+        std::vector<int> broke(count);
+        DH_CHECK_HIP(hipMemcpyAsync(broke.data(), P->sticky, sizeof(int) * count,
+                                    hipMemcpyDeviceToHost, h->stream));
+        DH_CHECK_HIP(hipStreamSynchronize(h->stream));
+        for (int i = 0; i < count; ++i) {
+            if (broke[i] == 0) continue;
+            const dh_lra_problem& p = problems[i];
+            st[i] = dh_lra(h, prec, p.n, p.d, p.l, p.q, p.X, p.ldx, p.y, p.gamma, p.Omega,
+                                     p.ldo, p.Q, p.ldq, p.W, p.ldw);
+            if (st[i] < 0) return st[i];
+        }
+    }
+    for (int i = 0; i < count; ++i) {
+        if (status) status[i] = st[i];
+        if (!first && st[i]) first = st[i];
+    }
+    DH_CHECK_HIP(hipStreamSynchronize(h->stream));
+    return first;
+}

@@ -34,7 +34,21 @@ struct dh_handle_s {
     void* ws[DH_WS_COUNT] = {};
     size_t ws_size[DH_WS_COUNT] = {};
     int num_cu = 0;
+    void* batch = nullptr;  // cached batched range-finder plan (batch.hip)
+    void* tune = nullptr;   // autotuner decisions (batch.hip)
 };
+
+// Release the batched plan / tuning table of a handle (batch.hip).
+void dh_batch_release(dh_handle_t h);
+void dh_tune_release(dh_handle_t h);
+// Autotuner table: choice for `key`, or -1 if not decided yet.
+int dh_tune_lookup(dh_handle_t h, uint64_t key);
+void dh_tune_store(dh_handle_t h, uint64_t key, int choice);
+// False when DH_TUNE=0 in the environment (fixed defaults everywhere).
+bool dh_tune_enabled();
+// Key of a BF16 projection schedule decision: tile class and row count
+// (bucketed by powers of two: the choice depends on how full the chip is).
+uint64_t dh_tune_key_bf16(int dk, int ncb, int waves, int rows);
 
 // Records a formatted message for dh_last_error_string and returns `code`.
 int dh_fail(int code, const char* fmt, ...);
@@ -74,3 +88,18 @@ int dh_potrf_impl(dh_handle_t h, int n, T* A, int lda, int bs, int* info_host);
 template <typename T>
 int dh_symm_upper_impl(dh_handle_t h, int m, int n, T alpha, const T* A, int lda, const T* B,
                        int ldb, T beta, T* C, int ldc);
+
+// One problem of a batched CholeskyQR step (small_spd.hip): Y (m x n, ld ldy)
+// is orthogonalised with the Cholesky factor of G (n x n, ld ldg, holding
+// Y^T Y on entry); Dinv receives 256 * ceil(n/16) doubles; *info the status
+// of this step, and *sticky is set to 1 (never cleared) when it is non-zero.
+struct DhSpdJob {
+    int m, n, ldy, ldg;
+    double* Y;
+    double* G;
+    double* Dinv;
+    int* info;
+    int* sticky;
+};
+int dh_cholqr_step_batched(dh_handle_t h, const DhSpdJob* jobs, int count, int max_n, int max_m,
+                           bool shift);

@@ -83,6 +83,8 @@ extern "C" int dh_destroy(dh_handle_t h) {
     if (!h) return 0;
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
+    dh_batch_release(h);
+    dh_tune_release(h);
     for (int s = 0; s < DH_WS_COUNT; ++s)
         if (h->ws[s]) (void)hipFree(h->ws[s]);
     if (h->own_stream) (void)hipStreamDestroy(h->stream);

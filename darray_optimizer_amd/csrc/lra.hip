@@ -15,6 +15,9 @@ int dh_project_f64_impl(dh_handle_t h, int m, int n, int d, int l, const double*
 int dh_lra_bf16_passes(dh_handle_t h, int n, int d, int l, int q, const double* X, int ldx,
                        const double* y, double gamma, const double* Omega, int ldo, double* Q,
                        int ldq, double* W, int ldw);
+int dh_lra_f16_passes(dh_handle_t h, int n, int d, int l, int q, const double* X, int ldx,
+                      const double* y, double gamma, const double* Omega, int ldo, double* Q,
+                      int ldq, double* W, int ldw);
 
 // dst (n x n, ld n) = triu(src); strict lower set to zero.
 __global__ __launch_bounds__(256) void triu_copy_kernel(int n, const double* __restrict__ src,
@@ -213,6 +216,39 @@ int dh_cholqr2_impl(dh_handle_t h, int m, int n, double* Y, int ldy, double* R, 
     return 0;
 }
 
+// col[i] = uniform pseudo-random in [-1, 1) from a counter hash (splitmix64).
+__global__ __launch_bounds__(256) void random_column_kernel(int m, double* __restrict__ col,
+                                                            uint64_t seed) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256) {
+        uint64_t z = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(i + 1);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        col[i] = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+    }
+}
+
+// Orthonormal basis of the range finder.  CholeskyQR2 (shifted CholeskyQR3
+// fallback) fails for Y of rank < n -- a kernel of rank < l, e.g. fewer
+// distinct points than basis columns -- where the reference's Householder
+// TSQR (lapack_like.h:281-379) would still return orthonormal columns.  Then
+// the column at the failing pivot is replaced by a pseudo-random vector and
+// the orthogonalisation repeated: the basis spans range(Y) plus random
+// directions, which can only enlarge the subspace the range finder keeps.
+int dh_range_orth(dh_handle_t h, int m, int n, double* Y, int ldy) {
+    for (int attempt = 0; attempt <= n; ++attempt) {
+        int st = dh_cholqr2_impl(h, m, n, Y, ldy, nullptr, 0);
+        if (st <= 0) return st;
+        int piv = st % 1000000;
+        if (piv < 1 || piv > n) return st;
+        uint64_t seed = 0xD1B54A32D192ED03ull * (uint64_t)(attempt + 1) + (uint64_t)piv;
+        random_column_kernel<<<max(1, min(dh_cdiv(m, 256), 1024)), 256, 0, h->stream>>>(
+            m, Y + (size_t)(piv - 1) * ldy, seed);
+        DH_CHECK_LAUNCH();
+    }
+    return dh_fail(n, "range finder: basis of %d columns could not be completed", n);
+}
+
 extern "C" int dh_cholqr2_f64(dh_handle_t h, int m, int n, double* Y, int ldy, double* R,
                               int ldr) {
     const char* name = "dh_cholqr2_f64";
@@ -244,11 +280,11 @@ static int lra_f64(dh_handle_t h, int n, int d, int l, int q, const double* X, i
     DH_CHECK_STATUS(dh_sqnorms_f64(h, n, d, X, ldx, nrm));
     DH_CHECK_STATUS(dh_project_f64_impl(h, n, n, d, l, X, ldx, X, ldx, nrm, nrm, y, y, gamma,
                                         Omega, ldo, 0.0, Q, ldq));
-    DH_CHECK_STATUS(dh_cholqr2_impl(h, n, l, Q, ldq, nullptr, 0));
+    DH_CHECK_STATUS(dh_range_orth(h, n, l, Q, ldq));
     for (int it = 0; it < q; ++it) {
         DH_CHECK_STATUS(dh_project_f64_impl(h, n, n, d, l, X, ldx, X, ldx, nrm, nrm, y, y, gamma,
                                             Q, ldq, 0.0, W, ldw));
-        DH_CHECK_STATUS(dh_cholqr2_impl(h, n, l, W, ldw, nullptr, 0));
+        DH_CHECK_STATUS(dh_range_orth(h, n, l, W, ldw));
         DH_CHECK_STATUS(copy_panel(h, n, l, W, ldw, Q, ldq));
     }
     return dh_project_f64_impl(h, n, n, d, l, X, ldx, X, ldx, nrm, nrm, y, y, gamma, Q, ldq, 0.0,
@@ -260,7 +296,8 @@ extern "C" int dh_lra(dh_handle_t h, int prec, int n, int d, int l, int q, const
                       double* Q, int ldq, double* W, int ldw) {
     const char* name = "dh_lra";
     if (!h) return dh_fail(-1, "%s: NULL handle", name);
-    if (prec != DH_PREC_F64 && prec != DH_PREC_BF16) return dh_fail(-2, "%s: bad prec %d", name, prec);
+    if (prec != DH_PREC_F64 && prec != DH_PREC_BF16 && prec != DH_PREC_F16)
+        return dh_fail(-2, "%s: bad prec %d", name, prec);
     if (n < 0) return dh_fail(-3, "%s: n < 0", name);
     if (d < 1) return dh_fail(-4, "%s: d < 1", name);
     if (l < 0 || l > n) return dh_fail(-5, "%s: need 0 <= l <= n", name);
@@ -273,8 +310,10 @@ extern "C" int dh_lra(dh_handle_t h, int prec, int n, int d, int l, int q, const
     int st;
     if (prec == DH_PREC_F64)
         st = lra_f64(h, n, d, l, q, X, ldx, y, gamma, Omega, ldo, Q, ldq, W, ldw);
-    else
+    else if (prec == DH_PREC_BF16)
         st = dh_lra_bf16_passes(h, n, d, l, q, X, ldx, y, gamma, Omega, ldo, Q, ldq, W, ldw);
+    else
+        st = dh_lra_f16_passes(h, n, d, l, q, X, ldx, y, gamma, Omega, ldo, Q, ldq, W, ldw);
     if (st) return st;
     DH_CHECK_HIP(hipStreamSynchronize(h->stream));
     return 0;

@@ -45,10 +45,9 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-__global__ __launch_bounds__(NT) void chol_small_kernel(int n, double* __restrict__ G, int ldg,
-                                                        double* __restrict__ Dinv, int m_rows,
-                                                        int shift, int rel_check,
-                                                        int* __restrict__ info) {
+__device__ __forceinline__ void chol_small_body(int n, double* __restrict__ G, int ldg,
+                                                double* __restrict__ Dinv, int m_rows, int shift,
+                                                int rel_check, int* __restrict__ info) {
     __shared__ double sD[NB][NB + 1];            // diagonal block, sD[j][i] = A(k0+i, k0+j)
     __shared__ __align__(16) double sP[NB][PW];  // row panel U12, sP[kk][c]
     __shared__ double red[2][NT / 64];
@@ -243,23 +242,36 @@ __global__ __launch_bounds__(NT) void chol_small_kernel(int n, double* __restric
     if (tid == 0) *info = 0;
 }
 
+__global__ __launch_bounds__(NT) void chol_small_kernel(int n, double* __restrict__ G, int ldg,
+                                                        double* __restrict__ Dinv, int m_rows,
+                                                        int shift, int rel_check,
+                                                        int* __restrict__ info) {
+    chol_small_body(n, G, ldg, Dinv, m_rows, shift, rel_check, info);
+}
+
+// Batched: workgroup b factors problem b of the table.
+__global__ __launch_bounds__(NT) void chol_small_batched_kernel(const DhSpdJob* __restrict__ jobs,
+                                                                int shift, int rel_check) {
+    const DhSpdJob& j = jobs[blockIdx.x];
+    chol_small_body(j.n, j.G, j.ldg, j.Dinv, j.m, shift, rel_check, j.info);
+    // Thread 0 wrote *j.info last inside the body.
+    if (threadIdx.x == 0 && *j.info != 0) *j.sticky = 1;
+}
+
 // Q = Y U^{-1} in place for the m x n panel Y (n <= 544), one wave per 16 rows.
 // The wave keeps its 16 solved rows in LDS (odd row pitch: conflict-free
 // A-operand reads) and for each 16-column block J accumulates
 //   acc = sum_{K<J} Q[:,K] U[K,J]      (f64 MFMA 16x16x4 steps, U from L2)
 // then Q_J = (Y_J - acc) U_JJ^{-1} (four more MFMAs, U_JJ^{-1} from Dinv).
 template <int WAVES>
-__global__ __launch_bounds__(WAVES * 64) void trsm_rows_kernel(int m, int n,
-                                                               double* __restrict__ Y, int ldy,
-                                                               const double* __restrict__ U,
-                                                               int ldu,
-                                                               const double* __restrict__ Dinv,
-                                                               int pitch) {
+__device__ __forceinline__ void trsm_rows_body(int blk, int m, int n, double* __restrict__ Y,
+                                               int ldy, const double* __restrict__ U, int ldu,
+                                               const double* __restrict__ Dinv, int pitch) {
     extern __shared__ double smem[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, li = lane & 15;
     double* Qs = smem + (size_t)wave * 16 * pitch;  // Qs[row * pitch + col]
-    const int r0 = (blockIdx.x * WAVES + wave) * 16;
+    const int r0 = (blk * WAVES + wave) * 16;
     if (r0 >= m) return;
     const int nblk = (n + 15) / 16;
     for (int J = 0; J < nblk; ++J) {
@@ -322,6 +334,27 @@ __global__ __launch_bounds__(WAVES * 64) void trsm_rows_kernel(int m, int n,
     }
 }
 
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void trsm_rows_kernel(int m, int n,
+                                                               double* __restrict__ Y, int ldy,
+                                                               const double* __restrict__ U,
+                                                               int ldu,
+                                                               const double* __restrict__ Dinv,
+                                                               int pitch) {
+    trsm_rows_body<WAVES>(blockIdx.x, m, n, Y, ldy, U, ldu, Dinv, pitch);
+}
+
+// Batched: blockIdx.y selects the problem; problems whose Cholesky failed
+// (*info != 0) are left untouched, as the unbatched sweep does.
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void trsm_rows_batched_kernel(
+    const DhSpdJob* __restrict__ jobs) {
+    const DhSpdJob& j = jobs[blockIdx.y];
+    if (blockIdx.x * WAVES * 16 >= (unsigned)j.m || *j.info != 0) return;
+    trsm_rows_body<WAVES>(blockIdx.x, j.m, j.n, j.Y, j.ldy, j.G, j.ldg, j.Dinv,
+                          16 * ((j.n + 15) / 16) + 1);
+}
+
 }  // namespace
 
 int dh_chol_small_max_order() { return kMaxN; }
@@ -333,6 +366,30 @@ int dh_chol_small_launch(dh_handle_t h, int n, double* G, int ldg, double* Dinv,
     if (n > kMaxN) return dh_fail(DH_ERR_UNSUPPORTED, "chol_small: order %d > %d", n, kMaxN);
     chol_small_kernel<<<1, NT, 0, h->stream>>>(n, G, ldg, Dinv, m_rows, shift ? 1 : 0,
                                                rel_check ? 1 : 0, info_dev);
+    DH_CHECK_LAUNCH();
+    return 0;
+}
+
+// Batched CholeskyQR step: for each job, G := U (Cholesky with the relative
+// breakdown test, optional shift), then Y := Y U^{-1} unless it broke down.
+int dh_cholqr_step_batched(dh_handle_t h, const DhSpdJob* jobs, int count, int max_n, int max_m,
+                           bool shift) {
+    if (max_n > kMaxN) return dh_fail(DH_ERR_UNSUPPORTED, "chol_small: order %d > %d", max_n, kMaxN);
+    constexpr int W = 2;
+    static bool lds_attr_set = false;  // set outside any graph capture: first call has count 0
+    if (!lds_attr_set) {
+        DH_CHECK_HIP(hipFuncSetAttribute((const void*)trsm_rows_batched_kernel<W>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)(sizeof(double) * 16 * (16 * 34 + 1) * W)));
+        lds_attr_set = true;
+    }
+    if (count == 0) return 0;
+    chol_small_batched_kernel<<<count, NT, 0, h->stream>>>(jobs, shift ? 1 : 0, 1);
+    DH_CHECK_LAUNCH();
+    int pitch = 16 * ((max_n + 15) / 16) + 1;
+    size_t lds = sizeof(double) * 16 * (size_t)pitch * W;
+    dim3 grid(dh_cdiv(max_m, 16 * W), count);
+    trsm_rows_batched_kernel<W><<<grid, W * 64, lds, h->stream>>>(jobs);
     DH_CHECK_LAUNCH();
     return 0;
 }

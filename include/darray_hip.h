@@ -96,6 +96,13 @@ int dh_rbf_project_f64(dh_handle_t h, int m, int n, int d, int l, const double* 
                        const double* Xj, int ldxj, const double* yi, const double* yj,
                        double gamma, const double* O, int ldo, double beta, double* Y, int ldy);
 
+/* As dh_rbf_project_bf16 with FP16 in place of BF16 (points also scaled by a
+ * power of two so that max |x - mean| lies in (1/2, 1]; one synchronisation):
+ * the same MFMA rate with 3 more mantissa bits for the points, K and O. */
+int dh_rbf_project_f16(dh_handle_t h, int m, int n, int d, int l, const double* Xi, int ldxi,
+                       const double* Xj, int ldxj, const double* yi, const double* yj,
+                       double gamma, const double* O, int ldo, double* Y, int ldy);
+
 /* Mixed-precision fused Y[m x l] = K(Xi,Xj) * O (Y overwritten): points centred by
  * the mean of Xj and rounded to BF16, FP32 squared norms of the rounded points,
  * BF16 MFMA distance with FP32 accumulation, K = exp2(..) on v_exp_f32 rounded
@@ -180,16 +187,54 @@ int dh_cholqr2_f64(dh_handle_t h, int m, int n, double* Y, int ldy, double* R, i
 /* Precision of the K-passes of dh_lra. */
 #define DH_PREC_F64 0  /* FP64 MFMA distance + projection, FP64 exp */
 #define DH_PREC_BF16 1 /* BF16 MFMA distance + projection, FP32 accumulate and exp */
+#define DH_PREC_F16 2  /* as DH_PREC_BF16 with FP16 operands (dh_rbf_project_f16) */
 
 /* Randomised range finder on the RBF kernel of one point set (DArray::LRA,
  * dkernel.h:12-46, and the orthogonalisation of testsvm.cpp:102-108, fused):
  *   Y = K*Omega; Q = CholQR2(Y); q times { Y = K*Q; Q = CholQR2(Y) }; W = K*Q
  * so B = Q^T K = W^T.  X d x n, y labels (n, may be NULL), Omega n x l.
  * Outputs Q (n x l, FP64) and W (n x l, FP64).  `prec` selects the K-pass
- * arithmetic; CholeskyQR2 is FP64 in both.  Synchronises. */
+ * arithmetic; CholeskyQR2 is FP64 in all.  If Y has rank < l (K of rank < l),
+ * where CholeskyQR cannot orthonormalise it, the deficient columns are
+ * replaced by pseudo-random ones, so Q always has l orthonormal columns
+ * spanning range(Y) and more (as the reference's Householder QR would).
+ * Synchronises. */
 int dh_lra(dh_handle_t h, int prec, int n, int d, int l, int q, const double* X, int ldx,
            const double* y, double gamma, const double* Omega, int ldo, double* Q, int ldq,
            double* W, int ldw);
+
+/* One problem of dh_lra_batched: the arguments of dh_lra after `prec`. */
+typedef struct dh_lra_problem {
+    int n, d, l, q;
+    const double* X;
+    int ldx;
+    const double* y;
+    double gamma;
+    const double* Omega;
+    int ldo;
+    double* Q;
+    int ldq;
+    double* W;
+    int ldw;
+} dh_lra_problem;
+
+/* dh_lra for `count` independent problems of any n, d <= 256 and l <= 272
+ * (one-vs-rest or per-gamma SVM problems; no reference counterpart: DArray
+ * runs one LRA per process, testsvm.cpp:86-120).  With DH_PREC_BF16 every
+ * stage runs as one launch over all problems -- point preparation, operand
+ * packing, CholeskyQR2 (batched Gram, one-workgroup Cholesky per problem,
+ * row-parallel solve) -- and the K-passes as persistent kernels, one per
+ * tile class, pulling (problem, row block) items from a queue longest
+ * problem first; the whole sequence is captured in a hipGraph on the first
+ * call and replayed while the problem table (sizes and pointers) stays the
+ * same.  Tile schedules come from the autotuner.  A problem whose
+ * CholeskyQR breaks down is redone alone by dh_lra (shifted CholeskyQR3,
+ * completion of rank-deficient bases).
+ * DH_PREC_F64 and DH_PREC_F16 run dh_lra per problem.  status (host, may be NULL) receives
+ * each problem's dh_lra status; returns 0 or the first non-zero status.
+ * All problems must share q.  Synchronises. */
+int dh_lra_batched(dh_handle_t h, int prec, int count, const dh_lra_problem* problems,
+                   int* status);
 
 /* FP64 error pass: err2[0] = ||K - Q W^T||_F^2, err2[1] = ||K||_F^2, where err2 is a
  * HOST pointer to 2 doubles.  K is regenerated on the fly from X.  This is the

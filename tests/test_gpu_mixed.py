@@ -25,16 +25,22 @@ def bf16(x):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def emulate(Xi, Xj, O, gamma, yi=None, yj=None):
+def f16(x):
+    """Round to the nearest IEEE half (ties to even), as float64."""
+    return np.asarray(x, dtype=np.float64).astype(np.float16).astype(np.float64)
+
+
+def emulate(Xi, Xj, O, gamma, yi=None, yj=None, rnd=bf16, scale=1.0):
     mu = Xj.mean(axis=0)
-    a = bf16(Xi - mu)
-    b = bf16(Xj - mu)
+    a = rnd((Xi - mu) * scale)
+    b = rnd((Xj - mu) * scale)
+    g = gamma / scale**2
     na = (a * a).sum(1)
     nb = (b * b).sum(1)
     d2 = np.maximum(na[:, None] + nb[None, :] - 2 * a @ b.T, 0)
-    K = bf16(np.exp(-gamma * d2))
+    K = rnd(np.exp(-g * d2))
     Ow = O * (yj[:, None] if yj is not None else 1.0)
-    Y = K @ bf16(Ow)
+    Y = K @ rnd(Ow)
     if yi is not None:
         Y = Y * yi[:, None]
     return Y
@@ -83,6 +89,46 @@ def test_rbf_project_bf16_self_tile(dev):
     assert rel(Yg, orc.rbf_project(X, X, O, 0.5 / d)) < 5e-3
 
 
+def f16_scale(Xi, Xj):
+    """Power of two 2^-e with max |x - mean| 2^-e in [1/2, 1), as the FP16 path picks it."""
+    mu = Xj.mean(axis=0)
+    mx = max(np.abs(Xi - mu).max(), np.abs(Xj - mu).max())
+    return 1.0 if mx == 0 else 2.0 ** -np.frexp(np.float32(mx))[1]
+
+
+@pytest.mark.parametrize("m,n,d,l", SHAPES)
+def test_rbf_project_f16(dev, m, n, d, l):
+    """FP16 operands: emulation to 1e-4, oracle to FP16 accuracy (1e-3, tighter than BF16)."""
+    import torch
+    rng = np.random.default_rng(7 * m + n + d + l)
+    Xi, Xj = rng.uniform(-1, 1, (m, d)) * 300.0, rng.uniform(-1, 1, (n, d)) * 300.0
+    yi, yj = rng.choice([-1.0, 1.0], m), rng.choice([-1.0, 1.0], n)
+    O = rng.standard_normal((n, l))
+    gamma = 1.0 / (d * 300.0**2)
+    Y = torch.empty((l, m), dtype=torch.float64, device="cuda")
+    dev.rbf_project_f16(dev_pts(Xi), dev_pts(Xj), dh.to_dev(O), gamma, Y, dev_vec(yi), dev_vec(yj))
+    Yg = dh.from_dev(Y)
+    sc = f16_scale(Xi, Xj)
+    assert rel(Yg, emulate(Xi, Xj, O, gamma, yi, yj, rnd=f16, scale=sc)) < 1e-4
+    assert rel(Yg, orc.rbf_project(Xi, Xj, O, gamma, yi, yj)) < 1e-3
+
+
+def test_rbf_project_f16_beats_bf16(dev):
+    import torch
+    n, d, l = 2048, 128, 272
+    X = dh.fill_mixture(3, d, n, 16, 0.25)
+    O = np.asfortranarray(dh.fill_normal(4, n, l))
+    Xd, Od = dev_pts(X), dh.to_dev(O)
+    ref = orc.rbf_project(X, X, O, 0.5 / d)
+    Yb = torch.empty((l, n), dtype=torch.float64, device="cuda")
+    Yh = torch.empty_like(Yb)
+    dev.rbf_project_bf16(Xd, Xd, Od, 0.5 / d, Yb)
+    dev.rbf_project_f16(Xd, Xd, Od, 0.5 / d, Yh)
+    eb, eh = rel(dh.from_dev(Yb), ref), rel(dh.from_dev(Yh), ref)
+    print(f"rel err vs FP64: bf16 {eb:.3e}  f16 {eh:.3e}")
+    assert eh < eb / 4
+
+
 @pytest.mark.parametrize("labels", [False, True])
 def test_lra_mixed_error_close_to_fp64(dev, labels):
     """Same seeded inputs through the FP64 and the mixed LRA: the mixed
@@ -94,7 +140,7 @@ def test_lra_mixed_error_close_to_fp64(dev, labels):
     Om = dh.fill_normal(6, n, l)
     Xd, Omd, yd = dev_pts(X), dh.to_dev(Om), dev_vec(y)
     errs = {}
-    for prec in (dh.DH_PREC_F64, dh.DH_PREC_BF16):
+    for prec in (dh.DH_PREC_F64, dh.DH_PREC_BF16, dh.DH_PREC_F16):
         Q = torch.empty((l, n), dtype=torch.float64, device="cuda")
         W = torch.empty((l, n), dtype=torch.float64, device="cuda")
         assert dev.lra(Xd, Omd, 1.0 / d, q, Q, W, yd, prec=prec) == 0
@@ -106,3 +152,4 @@ def test_lra_mixed_error_close_to_fp64(dev, labels):
     eo, ko = orc.lra_error(X, 1.0 / d, Qo, Wo, y)
     assert abs(errs[dh.DH_PREC_F64] - np.sqrt(eo / ko)) <= 1e-6 * np.sqrt(eo / ko) + 1e-12
     assert errs[dh.DH_PREC_BF16] <= 1.5 * errs[dh.DH_PREC_F64] + 3e-3
+    assert errs[dh.DH_PREC_F16] <= 1.2 * errs[dh.DH_PREC_F64] + 5e-4
