@@ -47,13 +47,13 @@ struct DhProjItem {
 struct DhBf16Class {
     int dk;     // 16-byte feature groups / 4 (dpad / 32)
     int ncb;    // 16-column blocks per workgroup
-    int waves;  // 8 or 4
-    int pp;     // ping-pong schedule
+    int rows;   // rows per work item: 256, or 128 for dk = 8
+    int wide;   // 4 waves x 64 rows instead of 8 waves x 32 rows (dk < 8)
 };
 
 int dh_bf16_dpad(int d);
-// Waves per workgroup the BF16 projection uses for dpad.
-int dh_bf16_waves(int dpad);
+// Rows of one BF16 projection workgroup for dpad (256, or 128 for dpad 256).
+int dh_bf16_rows(int dpad);
 // Column blocking of an l-wide pass: groups of ncb blocks, Op packed ncb_pack wide.
 void dh_bf16_columns(int l, int* ncb, int* groups, int* ncb_pack);
 size_t dh_bf16_packed_bytes(int n, int ncb_pack);

@@ -29,6 +29,7 @@ namespace {
 
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr int kGramK = 4096;  // rows of Y per split-K Gram workgroup
+#define DH_BF16_DEFAULT_WIDE 0      // schedule of a class when tuning is off
 
 // ------------------------------------------------------------------ kernels
 
@@ -165,6 +166,15 @@ bool dh_tune_enabled() {
     return on != 0;
 }
 
+int dh_forced_schedule() {
+    static int v = -2;
+    if (v == -2) {
+        const char* e = std::getenv("DH_SCHEDULE");
+        v = (e && e[0] >= '0' && e[0] <= '9') ? std::atoi(e) : -1;
+    }
+    return v;
+}
+
 int dh_tune_lookup(dh_handle_t h, uint64_t key) {
     if (!h->tune) return -1;
     auto& m = static_cast<TuneTable*>(h->tune)->choice;
@@ -189,9 +199,9 @@ void dh_batch_release(dh_handle_t h) {
 
 namespace {
 
-// Decide the ping-pong choice of each class by timing both schedules on the
-// first projection pass (identical results: only the wave schedule differs).
-// Runs eagerly, before the capture.
+// Decide the schedule of each class (8 waves x 32 rows or 4 waves x 64 rows)
+// by timing both on the first projection pass (identical results: only the
+// wave layout differs).  Runs eagerly, before the capture.
 int tune_classes(dh_handle_t h, std::vector<ClassRange>& classes, const DhProjJob* jobs,
                  const DhProjItem* items, int* counters) {
     hipEvent_t e0, e1;
@@ -200,42 +210,42 @@ int tune_classes(dh_handle_t h, std::vector<ClassRange>& classes, const DhProjJo
     int status = 0;
     for (size_t c = 0; c < classes.size() && !status; ++c) {
         ClassRange& cr = classes[c];
-        if (cr.cls.waves != 8) {
+        if (cr.cls.dk >= 8) {
             // One schedule only; run it once eagerly so that its one-time
             // launch set-up happens outside the graph capture.
-            cr.cls.pp = 0;
+            cr.cls.wide = 0;
             fill_int_kernel<<<1, 256, 0, h->stream>>>(counters, 1, 0);
             status = dh_bf16_project_persistent(h, cr.cls, jobs, items + cr.first, cr.count,
                                                 counters);
             continue;
         }
-        uint64_t key = dh_tune_key_bf16(cr.cls.dk, cr.cls.ncb, 8, cr.count * 256) ^ ((uint64_t)1 << 60);
+        uint64_t key = dh_tune_key_bf16(cr.cls.dk, cr.cls.ncb, 1, cr.count * 256) ^ ((uint64_t)1 << 60);
         int known = dh_tune_lookup(h, key);
-        if (known >= 0 || !dh_tune_enabled()) {
-            cr.cls.pp = known >= 0 ? known : 1;
-            continue;
-        }
-        float best = 1e30f;
-        int best_pp = 1;
-        for (int pp = 1; pp >= 0 && !status; --pp) {
+        if (known < 0 && !dh_tune_enabled()) known = DH_BF16_DEFAULT_WIDE;
+        // Candidates run once eagerly (their launch set-up then happens outside
+        // the capture); with no recorded choice the faster one is recorded.
+        const int first = known >= 0 ? known : 0, lastc = known >= 0 ? known : 1;
+        float best = 3.4e38f;
+        int best_wide = first;
+        for (int wide = first; wide <= lastc && !status; ++wide) {
             DhBf16Class k = cr.cls;
-            k.pp = pp;
+            k.wide = wide;
             fill_int_kernel<<<1, 256, 0, h->stream>>>(counters, 1, 0);
             if ((status = hipGetLastError() != hipSuccess ? DH_ERR_HIP : 0)) break;
             (void)hipEventRecord(e0, h->stream);
             status = dh_bf16_project_persistent(h, k, jobs, items + cr.first, cr.count, counters);
             (void)hipEventRecord(e1, h->stream);
-            if (!status && hipEventSynchronize(e1) == hipSuccess) {
+            if (!status && first != lastc && hipEventSynchronize(e1) == hipSuccess) {
                 float ms = 0.f;
                 (void)hipEventElapsedTime(&ms, e0, e1);
                 if (ms < best) {
                     best = ms;
-                    best_pp = pp;
+                    best_wide = wide;
                 }
             }
         }
-        cr.cls.pp = best_pp;
-        dh_tune_store(h, key, best_pp);
+        cr.cls.wide = best_wide;
+        if (first != lastc) dh_tune_store(h, key, best_wide);
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -250,7 +260,7 @@ int build_plan(dh_handle_t h, Plan& P, int prec, int count, const dh_lra_problem
 
     // ---- sizes and layout
     struct Inst {
-        int dpad, npad, nchunks, ncb, groups, ncb_pack, waves, rows, nsplit;
+        int dpad, npad, nchunks, ncb, groups, ncb_pack, rows, nsplit;
         size_t o_part, o_mean, o_xb, o_a, o_op, o_g, o_gpart, o_dinv;
     };
     std::vector<Inst> in(count);
@@ -272,8 +282,7 @@ int build_plan(dh_handle_t h, Plan& P, int prec, int count, const dh_lra_problem
         s.npad = dh_cdiv(p.n, 32) * 32;
         s.nchunks = dh_cdiv(p.n, 32);
         dh_bf16_columns(p.l, &s.ncb, &s.groups, &s.ncb_pack);
-        s.waves = dh_bf16_waves(s.dpad);
-        s.rows = dh_cdiv(p.n, s.waves * 32);
+        s.rows = dh_cdiv(p.n, dh_bf16_rows(s.dpad));
         s.nsplit = dh_cdiv(p.n, kGramK);
         s.o_part = cv.take(sizeof(double) * (size_t)dh_cdiv(p.n, 256) * p.d);
         s.o_mean = cv.take(sizeof(double) * p.d);
@@ -309,7 +318,7 @@ int build_plan(dh_handle_t h, Plan& P, int prec, int count, const dh_lra_problem
             std::stable_sort(ids.begin(), ids.end(),
                              [&](int a, int b) { return probs[a].n > probs[b].n; });
             ClassRange cr;
-            cr.cls = {kv.first.first, kv.first.second, dh_bf16_waves(kv.first.first * 32), 1};
+            cr.cls = {kv.first.first, kv.first.second, dh_bf16_rows(kv.first.first * 32), 0};
             cr.first = (int)items.size();
             for (int i : ids)
                 for (int r = 0; r < in[i].rows; ++r)

@@ -46,6 +46,9 @@ int dh_tune_lookup(dh_handle_t h, uint64_t key);
 void dh_tune_store(dh_handle_t h, uint64_t key, int choice);
 // False when DH_TUNE=0 in the environment (fixed defaults everywhere).
 bool dh_tune_enabled();
+// DH_SCHEDULE=<n> in the environment forces schedule n of the one-call 16-bit
+// projection (for measurements); -1 when unset.
+int dh_forced_schedule();
 // Key of a BF16 projection schedule decision: tile class and row count
 // (bucketed by powers of two: the choice depends on how full the chip is).
 uint64_t dh_tune_key_bf16(int dk, int ncb, int waves, int rows);

@@ -1,0 +1,389 @@
+// Fused 16-bit projection kernels: distance MFMA, exp epilogue, projection
+// MFMA (templates shared by the one-call path, the per-width dispatch units
+// rbf16_dk*.hip and the batched persistent kernels).
+//
+// Precision scheme (per pass):
+//   * X is centred by the mean of the column point set (distances are
+//     translation invariant) and rounded to BF16 (or FP16 after a power-of-two
+//     scaling) once per solve;
+//   * the squared norms are FP32 sums of the ROUNDED coordinates, so
+//     |xi|^2 + |xj|^2 - 2 xi.xj is the squared distance of the rounded
+//     points up to FP32 accumulation error;
+//   * K = exp2(-gamma log2(e) d2) on v_exp_f32, clamped to [0, 1] and rounded
+//     to BF16 in registers;
+//   * Omega (or Q) is rounded to BF16 with the labels y_j folded in, stored
+//     in the operand order of the projection MFMA ("packed");
+//   * Y accumulates in FP32 and is written as FP64 for CholeskyQR2.
+// Reference: DArray::LRA (dkernel.h:12-46) over rbf_kernel + tc_gemm
+// (dkernel.cu:132-231), whose tensor-core path is an FP16 GemmEx with FP32
+// accumulation (test_cuda.cu:111-132, dkernel.cu:161-163).
+#pragma once
+
+#include <cmath>
+#include <cstring>
+#include <type_traits>
+
+#include "dh_common.h"
+
+// 16-byte groups of eight 16-bit values.  The same kernels serve BF16 and
+// FP16 (template flag F16): the bits are stored as bf16x8 and reinterpreted
+// for the FP16 MFMA.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr float kLog2e = 1.4426950408889634f;
+
+template <bool F16>
+__device__ __forceinline__ __bf16 to16(float x) {
+    if constexpr (F16)
+        return __builtin_bit_cast(__bf16, (_Float16)x);
+    else
+        return (__bf16)x;
+}
+
+template <bool F16>
+__device__ __forceinline__ float from16(__bf16 v) {
+    if constexpr (F16)
+        return (float)__builtin_bit_cast(_Float16, v);
+    else
+        return (float)v;
+}
+
+template <bool F16>
+__device__ __forceinline__ f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) {
+    if constexpr (F16)
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a),
+                                                       __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// ------------------------------------------------------------ main kernel
+//
+// Workgroup = WAVES waves; wave w owns 16*RB rows i (RB 16-row MFMA blocks).
+// The j loop walks chunks of 32 points through a 3-deep LDS ring: step t
+// computes the distance tile and K of chunk t while the projection of chunk
+// t-1 runs and chunk t+1 is in flight, so one barrier per chunk suffices
+// and the loop body is a single basic block (look-ahead indices are
+// clamped instead of branched).  Staging uses direct
+// global-to-LDS loads (global_load_lds_dwordx4): no registers, no LDS
+// stores; each wave waits for its own loads (vmcnt) before the barrier that
+// publishes them.  Per chunk a wave:
+//   1. S^T = Xj_chunk * Xi^T: 2 x RB blocks of 16x16, DK MFMA 16x16x32 each;
+//      A operand (Xj rows) from LDS, B operand (Xi rows) from registers.  Lane
+//      L holds S^T[16 jb + 4 g + r][16 ib + li] (g = L/16, li = L%16).
+//   2. exp epilogue in registers; the 8 values of a lane (jb, r) become the
+//      8 16-bit A-operand elements e = 4 jb + r of the projection MFMA, i.e.
+//      k-slot 8 g + e <-> j = 16 (e/4) + 4 g + e%4 -- the order Op is packed in;
+//   3. Y[i, c] += K[i, j] Op[j, c]: NCB column blocks x RB row blocks, B operand
+//      = one 16-byte LDS read per column block shared by the RB row blocks.
+// Accumulators stay in registers for the whole j loop; Y is written once.
+//
+// The exponent is c2 s + (a_i + b_j) with s = x_i . x_j, c2 = 2 gamma log2(e)
+// and a = -gamma log2(e) |x|^2; the exp's clamp to [0, 1] stands in for
+// clamping d2 at 0.
+//
+// project_rows is the body of one workgroup (row block `rblk`, column group
+// `cg`); rbf_project_bf16_kernel maps the grid onto it, and the batched
+// persistent kernel hands it work items from a queue.
+
+// Padded LDS image of one chunk of Xj: 32 rows of XG 16-byte groups plus one
+// group of padding per row (conflict-free A-operand reads), rounded up to
+// whole 64-slot DMA instructions.
+template <int DK>
+struct XSlots {
+    static constexpr int XG = DK * 4;
+    static constexpr int XROW = XG + 1;
+    static constexpr int INSTR = (32 * XROW + 63) / 64;
+    static constexpr int SLOTS = INSTR * 64;
+};
+
+// 16-byte (or 4-byte) direct global -> LDS copy by one wave: lane L's value
+// lands at lds + L * size; lds must be the same for all lanes of the wave.
+// Written as inline assembly (M0 = LDS base, one wait state, then the load)
+// so that the compiler does not make every later LDS read wait for the copy:
+// the copies of a step go to a buffer nobody reads in that step, and
+// wait_vm() + the barrier at the end of the step publish them.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+__device__ __forceinline__ void dma16(const void* g, void* lds) {
+    unsigned base = (unsigned)(uintptr_t)(__attribute__((address_space(3))) void*)lds;
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
+                 :
+                 : "v"(g), "s"(__builtin_amdgcn_readfirstlane(base))
+                 : "memory", "m0");
+}
+__device__ __forceinline__ void dma4(const void* g, void* lds) {
+    unsigned base = (unsigned)(uintptr_t)(__attribute__((address_space(3))) void*)lds;
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off"
+                 :
+                 : "v"(g), "s"(__builtin_amdgcn_readfirstlane(base))
+                 : "memory", "m0");
+}
+#pragma clang diagnostic pop
+// Wait for this wave's outstanding global (and global -> LDS) loads only.
+__device__ __forceinline__ void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+template <int DK, int NCB, int WAVES, bool F16, int RB>
+__device__ __forceinline__ void project_rows(
+    int rblk, int cg, int m, int nchunks, int l, int ncb_total, const bf16x8* __restrict__ Xi,
+    const bf16x8* __restrict__ Xj, const float* __restrict__ ai, const float* __restrict__ bj,
+    float c2, float inv_c2, const bf16x8* __restrict__ Op, const double* __restrict__ yi,
+    double* __restrict__ Y, int ldy) {
+    using XS = XSlots<DK>;
+    constexpr int XG = XS::XG, XROW = XS::XROW;
+    constexpr int NBUF = 3;
+    __shared__ bf16x8 sX[NBUF][XS::SLOTS];
+    __shared__ bf16x8 sO[NBUF][NCB * 64];
+    __shared__ float sB[NBUF][64];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane >> 4, li = lane & 15;
+    const int cb0 = cg * NCB;
+    const int i0 = (rblk * WAVES + wave) * 16 * RB;
+
+    bf16x8 xb[RB][DK];
+    float arow[RB];
+#pragma unroll
+    for (int ib = 0; ib < RB; ++ib) {
+        int row = i0 + 16 * ib + li;
+#pragma unroll
+        for (int ks = 0; ks < DK; ++ks) {
+            bf16x8 z = {};
+            xb[ib][ks] = (row < m) ? Xi[(size_t)row * XG + ks * 4 + g] : z;
+        }
+        arow[ib] = (row < m) ? ai[row] : 0.f;
+    }
+    f32x4 acc[RB][NCB];
+#pragma unroll
+    for (int ib = 0; ib < RB; ++ib)
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) acc[ib][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    // Issue the loads of chunk t into buffer buf (instructions spread over the waves).
+    auto stage = [&](int t, int buf) {
+        const bf16x8* xc = Xj + (size_t)t * 32 * XG;
+#pragma unroll
+        for (int k = 0; k < XS::INSTR; k += WAVES) {
+            const int kk = k + wave;
+            if (XS::INSTR % WAVES == 0 || kk < XS::INSTR) {
+                int slot = kk * 64 + lane, row = slot / XROW, col = slot % XROW;
+                if (row >= 32 || col >= XG) {  // padding / rounding slots: never read
+                    row = 0;
+                    col = 0;
+                }
+                dma16(xc + row * XG + col, &sX[buf][kk * 64]);
+            }
+        }
+        const bf16x8* oc = Op + ((size_t)t * ncb_total + cb0) * 64;
+#pragma unroll
+        for (int k = 0; k < NCB; k += WAVES) {
+            const int kk = k + wave;
+            if (NCB % WAVES == 0 || kk < NCB) dma16(oc + kk * 64 + lane, &sO[buf][kk * 64]);
+        }
+        if (wave == WAVES - 1) dma4(bj + (size_t)t * 32 + (lane & 31), &sB[buf][0]);
+    };
+    auto distance = [&](int buf, f32x4 (&s)[2][RB]) {
+#pragma unroll
+        for (int jb = 0; jb < 2; ++jb)
+#pragma unroll
+            for (int ib = 0; ib < RB; ++ib) s[jb][ib] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < DK; ++ks)
+#pragma unroll
+            for (int jb = 0; jb < 2; ++jb) {
+                bf16x8 a = sX[buf][(16 * jb + li) * XROW + ks * 4 + g];
+#pragma unroll
+                for (int ib = 0; ib < RB; ++ib) s[jb][ib] = mfma16<F16>(a, xb[ib][ks], s[jb][ib]);
+            }
+    };
+    // K = clamp(exp2(c2 s + a_i + b_j), 0, 1) as the projection's A operand.
+    auto epilogue = [&](int buf, const f32x4 (&s)[2][RB], bf16x8 (&kb)[RB]) {
+#pragma unroll
+        for (int jb = 0; jb < 2; ++jb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                float bv = sB[buf][16 * jb + 4 * g + r];
+#pragma unroll
+                for (int ib = 0; ib < RB; ++ib) {
+                    float e = __builtin_amdgcn_exp2f(fmaf(c2, s[jb][ib][r], arow[ib] + bv));
+                    kb[ib][4 * jb + r] = to16<F16>(__builtin_amdgcn_fmed3f(e, 0.f, 1.f));
+                }
+            }
+    };
+    auto project = [&](int buf, const bf16x8 (&kb)[RB]) {
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) {
+            bf16x8 bo = sO[buf][cb * 64 + lane];
+#pragma unroll
+            for (int ib = 0; ib < RB; ++ib) acc[ib][cb] = mfma16<F16>(kb[ib], bo, acc[ib][cb]);
+        }
+    };
+
+    // Step t: distance of chunk t, then its epilogue next to the projection
+    // of chunk t-1 (independent work the scheduler interleaves); chunk t+1
+    // is in flight.  Buffers: chunk t in t%3 (read by distance and
+    // epilogue), chunk t-1 in (t-1)%3 (read by the projection), chunk t+1
+    // loading into (t+1)%3, whose last reader (the projection of chunk t-2)
+    // finished before the barrier that closed step t-1.
+    const int last = nchunks - 1;
+    stage(0, 0);
+    wait_vm();
+    __syncthreads();
+    f32x4 s[2][RB];
+    bf16x8 kb_prev[RB], kb[RB];
+    int buf = 0;  // buffer of chunk t
+    for (int t = 0; t < nchunks; ++t) {
+        const int nbuf = (buf == NBUF - 1) ? 0 : buf + 1;   // chunk t+1
+        const int pbuf = (buf == 0) ? NBUF - 1 : buf - 1;   // chunk t-1
+        stage(min(t + 1, last), nbuf);
+        distance(buf, s);
+        epilogue(buf, s, kb);
+        if (t > 0) project(pbuf, kb_prev);
+#pragma unroll
+        for (int ib = 0; ib < RB; ++ib) kb_prev[ib] = kb[ib];
+        wait_vm();
+        __syncthreads();
+        buf = nbuf;
+    }
+    project(buf == 0 ? NBUF - 1 : buf - 1, kb_prev);  // chunk last, after the final barrier
+
+#pragma unroll
+    for (int ib = 0; ib < RB; ++ib)
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int i = i0 + 16 * ib + 4 * g + r;
+                int c = 16 * (cb0 + cb) + li;
+                if (i < m && c < l) {
+                    double v = (double)acc[ib][cb][r];
+                    if (yi) v *= yi[i];
+                    Y[i + (size_t)c * ldy] = v;
+                }
+            }
+}
+
+template <int DK, int NCB, int WAVES, bool F16, int RB>
+__global__ __launch_bounds__(WAVES * 64, 1) void rbf_project_bf16_kernel(
+    int m, int nchunks, int l, int ncb_total, const bf16x8* __restrict__ Xi,
+    const bf16x8* __restrict__ Xj, const float* __restrict__ ai, const float* __restrict__ bj,
+    float c2, float inv_c2, const bf16x8* __restrict__ Op, const double* __restrict__ yi,
+    double* __restrict__ Y, int ldy) {
+    project_rows<DK, NCB, WAVES, F16, RB>(blockIdx.x, blockIdx.y, m, nchunks, l, ncb_total, Xi, Xj,
+                                          ai, bj, c2, inv_c2, Op, yi, Y, ldy);
+}
+
+// Schedule used when tuning is off (DH_TUNE=0) or during graph capture.
+#ifndef DH_BF16_DEFAULT_SCHEDULE
+#define DH_BF16_DEFAULT_SCHEDULE 0
+#endif
+
+// Schedules of one projection launch (same results, different wave layout):
+//   0: 8 waves x 32 rows (two waves per SIMD hide each other's latencies)
+//   1: 4 waves x 64 rows (four row blocks per wave share each LDS operand read)
+// and for d > 128, where the Xi fragments need the registers, 4 waves x 32 rows.
+// The autotuner's recorded choice for the shape is used; without one (and
+// outside graph capture, with tuning enabled) every candidate is timed on
+// this call's operands and the fastest recorded.
+template <int DK, int NCB, bool F16>
+int launch_bf16(dh_handle_t h, int m, int groups, int nchunks, int l, int ncb_total,
+                const bf16x8* Xi, const bf16x8* Xj, const float* ai, const float* bj, float c2,
+                const bf16x8* Op, const double* yi, double* Y, int ldy) {
+    const float inv_c2 = c2 > 0.f ? 1.f / c2 : 0.f;
+    auto run = [&](int sched) {
+#define DH_RUN(W, RB)                                                                        \
+    rbf_project_bf16_kernel<DK, NCB, W, F16, RB>                                             \
+        <<<dim3(dh_cdiv(m, W * 16 * RB), groups), W * 64, 0, h->stream>>>(                   \
+            m, nchunks, l, ncb_total, Xi, Xj, ai, bj, c2, inv_c2, Op, yi, Y, ldy)
+        if constexpr (DK >= 8) {
+            DH_RUN(4, 2);
+        } else {
+            if (sched == 0)
+                DH_RUN(8, 2);
+            else
+                DH_RUN(4, 4);
+        }
+#undef DH_RUN
+    };
+    constexpr int ncand = DK >= 8 ? 1 : 2;
+    const uint64_t key = dh_tune_key_bf16(DK, NCB, 0, m) ^ (F16 ? (uint64_t)1 << 61 : 0);
+    int sched = ncand == 1 ? 0 : dh_tune_lookup(h, key);
+    const int forced = dh_forced_schedule();
+    if (ncand > 1 && forced >= 0 && forced < ncand) sched = forced;
+    if (sched < 0) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        DH_CHECK_HIP(hipStreamIsCapturing(h->stream, &cs));
+        if (!dh_tune_enabled() || cs != hipStreamCaptureStatusNone) {
+            sched = DH_BF16_DEFAULT_SCHEDULE;
+        } else {
+            hipEvent_t ev[ncand + 1];
+            for (auto& e : ev) DH_CHECK_HIP(hipEventCreate(&e));
+            DH_CHECK_HIP(hipEventRecord(ev[0], h->stream));
+            for (int c = 0; c < ncand; ++c) {
+                run(c);
+                DH_CHECK_LAUNCH();
+                DH_CHECK_HIP(hipEventRecord(ev[c + 1], h->stream));
+            }
+            DH_CHECK_HIP(hipEventSynchronize(ev[ncand]));
+            float best = 3.4e38f;
+            for (int c = 0; c < ncand; ++c) {
+                float t = 0.f;
+                DH_CHECK_HIP(hipEventElapsedTime(&t, ev[c], ev[c + 1]));
+                if (t < best) {
+                    best = t;
+                    sched = c;
+                }
+            }
+            for (auto& e : ev) (void)hipEventDestroy(e);
+            dh_tune_store(h, key, sched);
+        }
+    }
+    run(sched);
+    DH_CHECK_LAUNCH();
+    return 0;
+}
+
+template <int DK, bool F16>
+int dispatch_bf16_ncb(dh_handle_t h, int ncb, int groups, int m, int nchunks, int l,
+                      int ncb_total, const bf16x8* Xi, const bf16x8* Xj, const float* ai,
+                      const float* bj, float c2, const bf16x8* Op, const double* yi, double* Y,
+                      int ldy) {
+#define DH_BF16_NCB(N) \
+    case N:            \
+        return launch_bf16<DK, N, F16>(h, m, groups, nchunks, l, ncb_total, Xi, Xj, ai, bj, c2, Op, yi, Y, ldy);
+    switch (ncb) {
+        DH_BF16_NCB(1)
+        DH_BF16_NCB(2)
+        DH_BF16_NCB(3)
+        DH_BF16_NCB(5)
+        DH_BF16_NCB(9)
+        DH_BF16_NCB(17)
+        default:
+            return dh_fail(DH_ERR_UNSUPPORTED, "rbf_project_bf16: no kernel for %d column blocks",
+                           ncb);
+    }
+#undef DH_BF16_NCB
+    DH_CHECK_LAUNCH();
+    return 0;
+}
+
+}  // namespace
+
+// Per-feature-width dispatch of one projection launch (rbf16_dk<DK>.hip):
+// BF16 or FP16 operands, column blocks ncb, `groups` column groups.
+int dh_project16_dk1(bool f16, dh_handle_t h, int ncb, int groups, int m, int nchunks, int l, int ncb_total,
+                      const bf16x8* Xi, const bf16x8* Xj, const float* ai, const float* bj, float c2,
+                      const bf16x8* Op, const double* yi, double* Y, int ldy);
+int dh_project16_dk2(bool f16, dh_handle_t h, int ncb, int groups, int m, int nchunks, int l, int ncb_total,
+                      const bf16x8* Xi, const bf16x8* Xj, const float* ai, const float* bj, float c2,
+                      const bf16x8* Op, const double* yi, double* Y, int ldy);
+int dh_project16_dk4(bool f16, dh_handle_t h, int ncb, int groups, int m, int nchunks, int l, int ncb_total,
+                      const bf16x8* Xi, const bf16x8* Xj, const float* ai, const float* bj, float c2,
+                      const bf16x8* Op, const double* yi, double* Y, int ldy);
+int dh_project16_dk8(bool f16, dh_handle_t h, int ncb, int groups, int m, int nchunks, int l, int ncb_total,
+                      const bf16x8* Xi, const bf16x8* Xj, const float* ai, const float* bj, float c2,
+                      const bf16x8* Op, const double* yi, double* Y, int ldy);

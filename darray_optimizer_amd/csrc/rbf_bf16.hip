@@ -1,60 +1,13 @@
-// Mixed-precision K-passes: BF16 MFMA distance and projection with FP32
-// accumulation and an FP32 exp epilogue; FP64 in and out.
-//
-// Precision scheme (per pass):
-//   * X is centred by the mean of the column point set (distances are
-//     translation invariant) and rounded to BF16 once per solve;
-//   * the squared norms are FP32 sums of the ROUNDED coordinates, so
-//     |xi|^2 + |xj|^2 - 2 xi.xj is the squared distance of the rounded
-//     points up to FP32 accumulation error;
-//   * K = exp2(-gamma log2(e) d2) on v_exp_f32, clamped to [0, 1] and rounded
-//     to BF16 in registers;
-//   * Omega (or Q) is rounded to BF16 with the labels y_j folded in, stored
-//     in the operand order of the projection MFMA ("packed");
-//   * Y accumulates in FP32 and is written as FP64 for CholeskyQR2.
+// Mixed-precision K-passes, host side and preparation kernels: column means,
+// 16-bit centred points with their squared norms, the packed projection
+// operand, the one-call entry points dh_rbf_project_bf16 / _f16, the 16-bit
+// passes of dh_lra, and the batched preparation launches of dh_lra_batched.
+// The fused projection kernels themselves are in rbf16_kernels.h.
 // Reference: DArray::LRA (dkernel.h:12-46) over rbf_kernel + tc_gemm
-// (dkernel.cu:132-231), whose tensor-core path is an FP16 GemmEx with FP32
-// accumulation (test_cuda.cu:111-132, dkernel.cu:161-163).
-#include <cmath>
-#include <cstring>
-
-#include "dh_common.h"
-
-// 16-byte groups of eight 16-bit values.  The same kernels serve BF16 and
-// FP16 (template flag F16): the bits are stored as bf16x8 and reinterpreted
-// for the FP16 MFMA.
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
+// (dkernel.cu:132-231).
+#include "rbf16_kernels.h"
 
 namespace {
-
-constexpr float kLog2e = 1.4426950408889634f;
-
-template <bool F16>
-__device__ __forceinline__ __bf16 to16(float x) {
-    if constexpr (F16)
-        return __builtin_bit_cast(__bf16, (_Float16)x);
-    else
-        return (__bf16)x;
-}
-
-template <bool F16>
-__device__ __forceinline__ float from16(__bf16 v) {
-    if constexpr (F16)
-        return (float)__builtin_bit_cast(_Float16, v);
-    else
-        return (float)v;
-}
-
-template <bool F16>
-__device__ __forceinline__ f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) {
-    if constexpr (F16)
-        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a),
-                                                       __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
-    else
-        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-}
 
 // ----------------------------------------------------------------- prep
 
@@ -175,342 +128,11 @@ __global__ __launch_bounds__(256) void pack_omega_kernel(int n, int l, int nchun
     pack_one<F16>((long)blockIdx.x * 256 + threadIdx.x, n, l, nchunks, ncb_total, O, ldo, y, Op);
 }
 
-// ------------------------------------------------------------ main kernel
-//
-// Workgroup = WAVES waves; wave w owns 32 rows i (two 16-row MFMA blocks).
-// The j loop walks chunks of 32 points through a 3-deep LDS ring: while
-// chunk t is projected, chunk t+1 is already resident (its distance tile is
-// computed in the same step) and chunk t+2 is being staged, so one barrier
-// per chunk suffices and the loop body is a single basic block (staging and
-// look-ahead indices are clamped instead of branched).  Per chunk a wave:
-//   1. S^T = Xj_chunk * Xi^T: 2 x 2 blocks of 16x16, DK MFMA 16x16x32 each;
-//      A operand (Xj rows) from LDS, B operand (Xi rows) from registers.  Lane
-//      L holds S^T[16 jb + 4 g + r][16 ib + li] (g = L/16, li = L%16).
-//   2. exp epilogue in registers; the 8 values of a lane (jb, r) become the
-//      8 BF16 A-operand elements e = 4 jb + r of the projection MFMA, i.e.
-//      k-slot 8 g + e <-> j = 16 (e/4) + 4 g + e%4 -- the order Op is packed in;
-//   3. Y[i, c] += K[i, j] Op[j, c]: NCB column blocks x 2 row blocks, B operand
-//      = one 16-byte LDS read per column block shared by both row blocks.
-// Step t runs the epilogue of chunk t next to the distance MFMAs of chunk t+1
-// (independent work the scheduler interleaves), then the projection of t.
-// Accumulators stay in registers for the whole j loop; Y is written once.
-//
-// The distance accumulators start at a_i / c2 = -|x_i|^2 / 2, so the
-// exponent is one FMA, c2 * s + b_j, and the exp's clamp to [0, 1] stands in
-// for clamping d2 at 0.
-//
-// PP (ping-pong): the waves form two groups, one wave of each per SIMD
-// (waves w and w + WAVES/2), running half a step apart.  Each step is split
-// by a barrier into V (epilogue of t + distance of t+1: VALU-heavy) and M
-// (projection of t: MFMA and LDS).  Group B starts one half-step late, so
-// on every SIMD one wave is in V while the other is in M.  With half-steps
-// h, group A runs V(t) at h = 2t and M(t) at 2t+1; group B runs V(t) at
-// 2t+1 and M(t) at 2t+2.  Both groups write the staging buffer of chunk
-// t+2 at h = 2t+1: the last read of that buffer (chunk t-1, B's M(t-1)) is
-// at 2t and its first read (A's V(t+1)) at 2t+2, so three buffers suffice.
-// Every wave executes the same number of barriers.
-//
-// project_rows is the body of one workgroup (row block `rblk`, column group
-// `cg`); rbf_project_bf16_kernel maps the grid onto it, and the batched
-// persistent kernel hands it work items from a queue.
-template <int DK, int NCB, int WAVES, bool PP, bool F16>
-__device__ __forceinline__ void project_rows(
-    int rblk, int cg, int m, int nchunks, int l, int ncb_total, const bf16x8* __restrict__ Xi,
-    const bf16x8* __restrict__ Xj, const float* __restrict__ ai, const float* __restrict__ bj,
-    float c2, float inv_c2, const bf16x8* __restrict__ Op, const double* __restrict__ yi,
-    double* __restrict__ Y, int ldy) {
-    constexpr int NT = WAVES * 64;
-    constexpr int XG = DK * 4;     // 16-byte groups per point row
-    constexpr int XROW = XG + 1;   // LDS row pitch (one group of padding)
-    constexpr int XGRP = 32 * XG;  // groups of Xj per chunk
-    constexpr int OGRP = NCB * 64; // groups of Op per chunk (this column range)
-    constexpr int XPT = (XGRP + NT - 1) / NT;
-    constexpr int OPT = (OGRP + NT - 1) / NT;
-    constexpr int NBUF = 3;
-    __shared__ bf16x8 sX[NBUF][32 * XROW];
-    __shared__ bf16x8 sO[NBUF][OGRP];
-    __shared__ float sB[NBUF][32];
-
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int g = lane >> 4, li = lane & 15;
-    const int cb0 = cg * NCB;
-    const int i0 = rblk * (WAVES * 32) + wave * 32;
-
-    bf16x8 xb[2][DK];
-    f32x4 sinit[2];
-#pragma unroll
-    for (int ib = 0; ib < 2; ++ib) {
-        int row = i0 + 16 * ib + li;
-#pragma unroll
-        for (int ks = 0; ks < DK; ++ks) {
-            bf16x8 z = {};
-            xb[ib][ks] = (row < m) ? Xi[(size_t)row * XG + ks * 4 + g] : z;
-        }
-        float a0 = (row < m) ? ai[row] * inv_c2 : 0.f;
-        sinit[ib] = f32x4{a0, a0, a0, a0};
-    }
-    f32x4 acc[2][NCB];
-#pragma unroll
-    for (int ib = 0; ib < 2; ++ib)
-#pragma unroll
-        for (int cb = 0; cb < NCB; ++cb) acc[ib][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-
-    bf16x8 rx[XPT], ro[OPT];
-    float rb = 0.f;
-    auto gload = [&](int t) {
-#pragma unroll
-        for (int u = 0; u < XPT; ++u) {
-            int e = tid + u * NT;
-            if (XGRP % NT == 0 || e < XGRP) rx[u] = Xj[(size_t)t * XGRP + e];
-        }
-#pragma unroll
-        for (int u = 0; u < OPT; ++u) {
-            int e = tid + u * NT;
-            if (OGRP % NT == 0 || e < OGRP) ro[u] = Op[((size_t)t * ncb_total + cb0) * 64 + e];
-        }
-        if (tid < 32) rb = bj[t * 32 + tid];
-    };
-    auto sstore = [&](int buf) {
-#pragma unroll
-        for (int u = 0; u < XPT; ++u) {
-            int e = tid + u * NT;
-            if (XGRP % NT == 0 || e < XGRP) sX[buf][(e / XG) * XROW + (e % XG)] = rx[u];
-        }
-#pragma unroll
-        for (int u = 0; u < OPT; ++u) {
-            int e = tid + u * NT;
-            if (OGRP % NT == 0 || e < OGRP) sO[buf][e] = ro[u];
-        }
-        if (tid < 32) sB[buf][tid] = rb;
-    };
-    auto distance = [&](int buf, f32x4 (&s)[2][2]) {
-#pragma unroll
-        for (int jb = 0; jb < 2; ++jb)
-#pragma unroll
-            for (int ib = 0; ib < 2; ++ib) s[jb][ib] = sinit[ib];
-#pragma unroll
-        for (int ks = 0; ks < DK; ++ks)
-#pragma unroll
-            for (int jb = 0; jb < 2; ++jb) {
-                bf16x8 a = sX[buf][(16 * jb + li) * XROW + ks * 4 + g];
-#pragma unroll
-                for (int ib = 0; ib < 2; ++ib) s[jb][ib] = mfma16<F16>(a, xb[ib][ks], s[jb][ib]);
-            }
-    };
-    // K = clamp(exp2(c2 s + b_j), 0, 1) as the projection's A operand.
-    auto epilogue = [&](int buf, const f32x4 (&s)[2][2], bf16x8 (&kb)[2]) {
-#pragma unroll
-        for (int jb = 0; jb < 2; ++jb)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                float bv = sB[buf][16 * jb + 4 * g + r];
-#pragma unroll
-                for (int ib = 0; ib < 2; ++ib) {
-                    float e = __builtin_amdgcn_exp2f(fmaf(c2, s[jb][ib][r], bv));
-                    kb[ib][4 * jb + r] = to16<F16>(__builtin_amdgcn_fmed3f(e, 0.f, 1.f));
-                }
-            }
-    };
-    auto project = [&](int buf, const bf16x8 (&kb)[2]) {
-#pragma unroll
-        for (int cb = 0; cb < NCB; ++cb) {
-            bf16x8 bo = sO[buf][cb * 64 + lane];
-#pragma unroll
-            for (int ib = 0; ib < 2; ++ib) acc[ib][cb] = mfma16<F16>(kb[ib], bo, acc[ib][cb]);
-        }
-    };
-
-    // Prologue: chunks 0 and 1 resident, distance tile of chunk 0 computed.
-    const int last = nchunks - 1;
-    gload(0);
-    sstore(0);
-    gload(min(1, last));
-    sstore(1);
-    __syncthreads();
-    f32x4 s_cur[2][2], s_next[2][2];
-    bf16x8 kb[2];
-    distance(0, s_cur);
-    int buf = 0;  // buffer of chunk t
-    if (!PP) {
-        for (int t = 0; t < nchunks; ++t) {
-            const int nbuf = (buf == NBUF - 1) ? 0 : buf + 1;    // chunk t+1
-            const int sbuf = (nbuf == NBUF - 1) ? 0 : nbuf + 1;  // chunk t+2 (being staged)
-            gload(min(t + 2, last));
-            epilogue(buf, s_cur, kb);
-            distance(nbuf, s_next);
-            project(buf, kb);
-            sstore(sbuf);
-            __syncthreads();
-#pragma unroll
-            for (int jb = 0; jb < 2; ++jb)
-#pragma unroll
-                for (int ib = 0; ib < 2; ++ib) s_cur[jb][ib] = s_next[jb][ib];
-            buf = nbuf;
-        }
-    } else if (wave < WAVES / 2) {  // group A
-        for (int t = 0; t < nchunks; ++t) {
-            const int nbuf = (buf == NBUF - 1) ? 0 : buf + 1;
-            const int sbuf = (nbuf == NBUF - 1) ? 0 : nbuf + 1;
-            gload(min(t + 2, last));
-            epilogue(buf, s_cur, kb);
-            distance(nbuf, s_next);
-            __syncthreads();  // end of V(t), h = 2t
-            project(buf, kb);
-            sstore(sbuf);
-            __syncthreads();  // end of M(t), h = 2t+1
-#pragma unroll
-            for (int jb = 0; jb < 2; ++jb)
-#pragma unroll
-                for (int ib = 0; ib < 2; ++ib) s_cur[jb][ib] = s_next[jb][ib];
-            buf = nbuf;
-        }
-        __syncthreads();  // matches group B's leading half-step
-    } else {  // group B
-        gload(min(2, last));
-        __syncthreads();  // h = 0: start half a step behind
-        for (int t = 0; t < nchunks; ++t) {
-            const int nbuf = (buf == NBUF - 1) ? 0 : buf + 1;
-            const int sbuf = (nbuf == NBUF - 1) ? 0 : nbuf + 1;
-            epilogue(buf, s_cur, kb);
-            distance(nbuf, s_next);
-            sstore(sbuf);
-            __syncthreads();  // end of V(t), h = 2t+1
-            gload(min(t + 3, last));
-            project(buf, kb);
-            __syncthreads();  // end of M(t), h = 2t+2
-#pragma unroll
-            for (int jb = 0; jb < 2; ++jb)
-#pragma unroll
-                for (int ib = 0; ib < 2; ++ib) s_cur[jb][ib] = s_next[jb][ib];
-            buf = nbuf;
-        }
-    }
-
-#pragma unroll
-    for (int ib = 0; ib < 2; ++ib)
-#pragma unroll
-        for (int cb = 0; cb < NCB; ++cb)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                int i = i0 + 16 * ib + 4 * g + r;
-                int c = 16 * (cb0 + cb) + li;
-                if (i < m && c < l) {
-                    double v = (double)acc[ib][cb][r];
-                    if (yi) v *= yi[i];
-                    Y[i + (size_t)c * ldy] = v;
-                }
-            }
-}
-
-template <int DK, int NCB, int WAVES, bool PP, bool F16>
-__global__ __launch_bounds__(WAVES * 64, 1) void rbf_project_bf16_kernel(
-    int m, int nchunks, int l, int ncb_total, const bf16x8* __restrict__ Xi,
-    const bf16x8* __restrict__ Xj, const float* __restrict__ ai, const float* __restrict__ bj,
-    float c2, float inv_c2, const bf16x8* __restrict__ Op, const double* __restrict__ yi,
-    double* __restrict__ Y, int ldy) {
-    project_rows<DK, NCB, WAVES, PP, F16>(blockIdx.x, blockIdx.y, m, nchunks, l, ncb_total, Xi, Xj,
-                                          ai, bj, c2, inv_c2, Op, yi, Y, ldy);
-}
-
-// Waves per workgroup: 8 (two per SIMD, 256 registers each) while the Xi
-// fragments are small; 4 (one per SIMD, 512 registers) for d > 128.  The
-// ping-pong schedule needs two waves per SIMD.
-template <int DK>
-constexpr int waves_for() { return DK >= 8 ? 4 : 8; }
-
-#ifndef DH_BF16_PINGPONG
-#define DH_BF16_PINGPONG 1
-#endif
-
-// Ping-pong or plain schedule for this shape: the autotuner's recorded
-// choice, else (outside graph capture, tuning enabled) time both on this
-// call's operands -- they produce identical results -- and record the faster.
-template <int DK, int NCB, int W, bool F16>
-int launch_tuned(dh_handle_t h, dim3 grid, int m, int nchunks, int l, int ncb_total,
-                 const bf16x8* Xi, const bf16x8* Xj, const float* ai, const float* bj, float c2,
-                 float inv_c2, const bf16x8* Op, const double* yi, double* Y, int ldy) {
-    auto run = [&](bool pp) {
-        if (pp)
-            rbf_project_bf16_kernel<DK, NCB, W, true, F16><<<grid, W * 64, 0, h->stream>>>(
-                m, nchunks, l, ncb_total, Xi, Xj, ai, bj, c2, inv_c2, Op, yi, Y, ldy);
-        else
-            rbf_project_bf16_kernel<DK, NCB, W, false, F16><<<grid, W * 64, 0, h->stream>>>(
-                m, nchunks, l, ncb_total, Xi, Xj, ai, bj, c2, inv_c2, Op, yi, Y, ldy);
-    };
-    const uint64_t key = dh_tune_key_bf16(DK, NCB, W, m) ^ (F16 ? (uint64_t)1 << 61 : 0);
-    int pp = dh_tune_lookup(h, key);
-    if (pp < 0) {
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        DH_CHECK_HIP(hipStreamIsCapturing(h->stream, &cs));
-        if (!dh_tune_enabled() || cs != hipStreamCaptureStatusNone) {
-            pp = DH_BF16_PINGPONG;
-        } else {
-            hipEvent_t ev[3];
-            for (auto& e : ev) DH_CHECK_HIP(hipEventCreate(&e));
-            DH_CHECK_HIP(hipEventRecord(ev[0], h->stream));
-            run(true);
-            DH_CHECK_HIP(hipEventRecord(ev[1], h->stream));
-            run(false);
-            DH_CHECK_HIP(hipEventRecord(ev[2], h->stream));
-            DH_CHECK_HIP(hipEventSynchronize(ev[2]));
-            float t_pp = 0.f, t_plain = 0.f;
-            DH_CHECK_HIP(hipEventElapsedTime(&t_pp, ev[0], ev[1]));
-            DH_CHECK_HIP(hipEventElapsedTime(&t_plain, ev[1], ev[2]));
-            for (auto& e : ev) (void)hipEventDestroy(e);
-            pp = t_pp <= t_plain ? 1 : 0;
-            dh_tune_store(h, key, pp);
-        }
-    }
-    run(pp != 0);
-    DH_CHECK_LAUNCH();
-    return 0;
-}
-
-template <int DK, int NCB, bool F16>
-int launch_bf16(dh_handle_t h, int m, int groups, int nchunks, int l, int ncb_total,
-                const bf16x8* Xi, const bf16x8* Xj, const float* ai, const float* bj, float c2,
-                const bf16x8* Op, const double* yi, double* Y, int ldy) {
-    constexpr int W = waves_for<DK>();
-    const float inv_c2 = c2 > 0.f ? 1.f / c2 : 0.f;
-    dim3 grid(dh_cdiv(m, W * 32), groups);
-    if (W == 8)
-        return launch_tuned<DK, NCB, W, F16>(h, grid, m, nchunks, l, ncb_total, Xi, Xj, ai, bj, c2,
-                                             inv_c2, Op, yi, Y, ldy);
-    rbf_project_bf16_kernel<DK, NCB, W, false, F16><<<grid, W * 64, 0, h->stream>>>(
-        m, nchunks, l, ncb_total, Xi, Xj, ai, bj, c2, inv_c2, Op, yi, Y, ldy);
-    DH_CHECK_LAUNCH();
-    return 0;
-}
-
 int ncb_instance_bf16(int want) {
     static const int inst[] = {1, 2, 3, 5, 9, 17};
     for (int v : inst)
         if (v >= want) return v;
     return 17;
-}
-
-template <int DK, bool F16>
-int dispatch_bf16_ncb(dh_handle_t h, int ncb, int groups, int m, int nchunks, int l,
-                      int ncb_total, const bf16x8* Xi, const bf16x8* Xj, const float* ai,
-                      const float* bj, float c2, const bf16x8* Op, const double* yi, double* Y,
-                      int ldy) {
-#define DH_BF16_NCB(N) \
-    case N:            \
-        return launch_bf16<DK, N, F16>(h, m, groups, nchunks, l, ncb_total, Xi, Xj, ai, bj, c2, Op, yi, Y, ldy);
-    switch (ncb) {
-        DH_BF16_NCB(1)
-        DH_BF16_NCB(2)
-        DH_BF16_NCB(3)
-        DH_BF16_NCB(5)
-        DH_BF16_NCB(9)
-        DH_BF16_NCB(17)
-        default:
-            return dh_fail(DH_ERR_UNSUPPORTED, "rbf_project_bf16: no kernel for %d column blocks",
-                           ncb);
-    }
-#undef DH_BF16_NCB
-    DH_CHECK_LAUNCH();
-    return 0;
 }
 
 // Low-precision copy of one point set inside a workspace.
@@ -591,16 +213,16 @@ int project_16(dh_handle_t h, int m, int n, int d, int l, const LowpPoints& pi,
     const bf16x8* xj = (const bf16x8*)pj.xb;
     switch (dp) {
         case 32:
-            return dispatch_bf16_ncb<1, F16>(h, cols.ncb, grid, m, nchunks, l, cols.ncb_pack, xi, xj,
+            return dh_project16_dk1(F16, h, cols.ncb, grid, m, nchunks, l, cols.ncb_pack, xi, xj,
                                         pi.a, pj.a, c2, Op, yi, Y, ldy);
         case 64:
-            return dispatch_bf16_ncb<2, F16>(h, cols.ncb, grid, m, nchunks, l, cols.ncb_pack, xi, xj,
+            return dh_project16_dk2(F16, h, cols.ncb, grid, m, nchunks, l, cols.ncb_pack, xi, xj,
                                         pi.a, pj.a, c2, Op, yi, Y, ldy);
         case 128:
-            return dispatch_bf16_ncb<4, F16>(h, cols.ncb, grid, m, nchunks, l, cols.ncb_pack, xi, xj,
+            return dh_project16_dk4(F16, h, cols.ncb, grid, m, nchunks, l, cols.ncb_pack, xi, xj,
                                         pi.a, pj.a, c2, Op, yi, Y, ldy);
         default:
-            return dispatch_bf16_ncb<8, F16>(h, cols.ncb, grid, m, nchunks, l, cols.ncb_pack, xi, xj,
+            return dh_project16_dk8(F16, h, cols.ncb, grid, m, nchunks, l, cols.ncb_pack, xi, xj,
                                         pi.a, pj.a, c2, Op, yi, Y, ldy);
     }
 }
@@ -802,68 +424,10 @@ __global__ __launch_bounds__(256) void pack_batched_kernel(const DhPackJob* __re
                     j.ncb_pack, j.src, j.lds, j.y, (bf16x8*)j.Op);
 }
 
-// Persistent workgroups: each takes the next item from *counter until the
-// queue is empty; every wave of the workgroup sees the same item and leaves
-// the loop together.
-template <int DK, int NCB, int WAVES, bool PP>
-__global__ __launch_bounds__(WAVES * 64, 1) void rbf_project_bf16_persistent(
-    const DhProjJob* __restrict__ jobs, const DhProjItem* __restrict__ items, int nitems,
-    int* __restrict__ counter) {
-    __shared__ int s_item;
-    for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
-        __syncthreads();
-        const int it = s_item;
-        __syncthreads();
-        if (it >= nitems) break;
-        const DhProjItem w = items[it];
-        const DhProjJob& j = jobs[w.job];
-        project_rows<DK, NCB, WAVES, PP, false>(w.rblk, w.cg, j.m, j.nchunks, j.l, j.ncb_pack,
-                                         (const bf16x8*)j.xb, (const bf16x8*)j.xb, j.a, j.a, j.c2,
-                                         j.inv_c2, (const bf16x8*)j.Op, j.y, j.Y, j.ldy);
-    }
-}
-
-template <int DK, int NCB, int W, bool PP>
-int launch_persistent(dh_handle_t h, const DhProjJob* jobs, const DhProjItem* items, int nitems,
-                      int* counter) {
-    auto kern = rbf_project_bf16_persistent<DK, NCB, W, PP>;
-    static int per_cu = 0;  // resident workgroups per CU, queried once
-    if (per_cu == 0) {
-        DH_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, W * 64, 0));
-        per_cu = max(per_cu, 1);
-    }
-    int grid = max(1, min(nitems, per_cu * h->num_cu));
-    kern<<<grid, W * 64, 0, h->stream>>>(jobs, items, nitems, counter);
-    DH_CHECK_LAUNCH();
-    return 0;
-}
-
-template <int DK, int W>
-int persistent_ncb(dh_handle_t h, int ncb, bool pp, const DhProjJob* jobs, const DhProjItem* items,
-                   int nitems, int* counter) {
-#define DH_PERS(N)                                                                          \
-    case N:                                                                                 \
-        return pp ? launch_persistent<DK, N, W, (W == 8)>(h, jobs, items, nitems, counter)  \
-                  : launch_persistent<DK, N, W, false>(h, jobs, items, nitems, counter);
-    switch (ncb) {
-        DH_PERS(1)
-        DH_PERS(2)
-        DH_PERS(3)
-        DH_PERS(5)
-        DH_PERS(9)
-        DH_PERS(17)
-        default:
-            return dh_fail(DH_ERR_UNSUPPORTED, "rbf_project_bf16: no kernel for %d column blocks",
-                           ncb);
-    }
-#undef DH_PERS
-}
-
 }  // namespace
 
 int dh_bf16_dpad(int d) { return dpad_of(d); }
-int dh_bf16_waves(int dpad) { return dpad > 128 ? 4 : 8; }
+int dh_bf16_rows(int dpad) { return dpad > 128 ? 128 : 256; }
 
 void dh_bf16_columns(int l, int* ncb, int* groups, int* ncb_pack) {
     Bf16Cols c = bf16_cols(l);
@@ -895,15 +459,3 @@ int dh_bf16_pack_batched(dh_handle_t h, const DhPackJob* jobs, int count, long m
     return 0;
 }
 
-int dh_bf16_project_persistent(dh_handle_t h, DhBf16Class c, const DhProjJob* jobs,
-                               const DhProjItem* items, int nitems, int* counter) {
-    if (nitems == 0) return 0;
-    bool pp = c.pp != 0;
-    switch (c.dk) {
-        case 1: return persistent_ncb<1, 8>(h, c.ncb, pp, jobs, items, nitems, counter);
-        case 2: return persistent_ncb<2, 8>(h, c.ncb, pp, jobs, items, nitems, counter);
-        case 4: return persistent_ncb<4, 8>(h, c.ncb, pp, jobs, items, nitems, counter);
-        case 8: return persistent_ncb<8, 4>(h, c.ncb, false, jobs, items, nitems, counter);
-        default: return dh_fail(DH_ERR_UNSUPPORTED, "rbf_project_bf16: feature blocks %d", c.dk);
-    }
-}
