@@ -28,6 +28,18 @@ PEAK_FP64_MFMA = 78.6e12  # MI355X dense FP64 matrix, FLOP/s
 PEAK_BF16_MFMA = 2.5e15  # MI355X dense BF16 matrix, FLOP/s
 
 
+def measured_traffic(path):
+    """HBM bytes per launch of the profiled kernel, from the committed rocprofv3
+    counter summary (line 'traffic_bytes_per_launch: N'), or None."""
+    try:
+        for line in Path(path).read_text().splitlines():
+            if line.strip().startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def lra_flops(n, d, l, q):
     f_mfma = (q + 2) * (2.0 * n * n * d + 2.0 * n * n * l)
     f_orth = (q + 1) * (6.0 * n * l * l + 2.0 / 3.0 * l**3)
@@ -149,7 +161,7 @@ def main(argv=None) -> int:
     ap.add_argument("--k", type=int, default=256)
     ap.add_argument("--p", type=int, default=16)
     ap.add_argument("--q", type=int, default=2)
-    ap.add_argument("--cpu-rows", type=int, default=16384)
+    ap.add_argument("--cpu-rows", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--core", type=int, default=16384, help="core order (0: skip)")
     ap.add_argument("--nrhs", type=int, default=64)
@@ -202,27 +214,29 @@ def main(argv=None) -> int:
         res[name] = r
     mixed, fp64, f16 = res["mixed"], res["fp64"], res["f16"]
 
-    # Dominant kernel of each path: one fused K*Omega pass, timed with events
-    # on the stream it runs on (the handle runs on torch's current stream).
+    # Dominant kernel of each path: the fused K*Omega projection kernel alone,
+    # timed by the library with HIP events on the stream it is launched on.
     Yd = torch.empty((l, n), dtype=torch.float64, device="cuda")
     pass_flops = 2.0 * n * n * d + 2.0 * n * n * l
 
-    def pass_time(fn, reps):
-        fn()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
+    def kernel_time(fn, reps):
+        fn()  # the first call also settles the autotuner's schedule
+        h.timing(True)
+        h.timing_read()
         for _ in range(reps):
             fn()
-        e1.record(stream)
-        e1.synchronize()
-        return e0.elapsed_time(e1) / 1e3 / reps
+        ms, launches = h.timing_read()
+        h.timing(False)
+        return ms / 1e3 / launches
 
-    t_bf16 = pass_time(lambda: h.rbf_project_bf16(Xd, Xd, Omd, gamma, Yd), 10)
-    t_f64 = pass_time(lambda: h.rbf_project_f64(Xd, Xd, Omd, gamma, Yd), 2)
+    t_bf16 = kernel_time(lambda: h.rbf_project_bf16(Xd, Xd, Omd, gamma, Yd), 10)
+    t_f64 = kernel_time(lambda: h.rbf_project_f64(Xd, Xd, Omd, gamma, Yd), 2)
     roofline = {"bound": "mfma", "achieved": pass_flops / t_bf16 / 1e12,
                 "peak": PEAK_BF16_MFMA / 1e12, "unit": "TFLOP/s",
-                "frac": pass_flops / t_bf16 / PEAK_BF16_MFMA, "traffic": None,
-                "kernel": "rbf_project_bf16_kernel<4,17,8>", "kernel_ms": t_bf16 * 1e3}
+                "frac": pass_flops / t_bf16 / PEAK_BF16_MFMA,
+                "traffic": measured_traffic(ROOT / "profiles" / "rbf_project_bf16_n65536_d128_l272.md"),
+                "kernel": "rbf_project_bf16_kernel<4,17,8>", "kernel_ms": t_bf16 * 1e3,
+                "flops_per_launch": pass_flops}
     fp64["roofline"] = {"bound": "mfma", "achieved": pass_flops / t_f64 / 1e12,
                         "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s",
                         "frac": pass_flops / t_f64 / PEAK_FP64_MFMA,

@@ -76,6 +76,16 @@ class Handle:
     def synchronize(self):
         call("dh_synchronize", self.h)
 
+    def timing(self, on: bool):
+        """Record HIP events around the fused projection kernels (dh_timing_enable)."""
+        call("dh_timing_enable", self.h, 1 if on else 0)
+
+    def timing_read(self):
+        """(summed kernel milliseconds, launches) since the previous read."""
+        ms, n = C.c_double(), C.c_int()
+        call("dh_timing_read", self.h, C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
     def workspace_bytes(self) -> int:
         return lib().dh_workspace_bytes(self.h)
 

@@ -28,6 +28,8 @@ SIGNATURES: dict[str, tuple] = {
     "dh_synchronize": (_i, [_p]),
     "dh_last_error_string": (C.c_char_p, []),
     "dh_workspace_bytes": (_sz, [_p]),
+    "dh_timing_enable": (_i, [_p, _i]),
+    "dh_timing_read": (_i, [_p, C.POINTER(_d), C.POINTER(_i)]),
     "dh_malloc": (_i, [_p, _sz, C.POINTER(_p)]),
     "dh_free": (_i, [_p, _p]),
     "dh_copy_2d": (_i, [_p, _i, _i, _i, _sz, _p, _i, _p, _i]),

@@ -36,7 +36,13 @@ struct dh_handle_s {
     int num_cu = 0;
     void* batch = nullptr;  // cached batched range-finder plan (batch.hip)
     void* tune = nullptr;   // autotuner decisions (batch.hip)
+    void* timing = nullptr; // kernel timing events when enabled (handle.hip)
 };
+
+// Bracket one fused projection kernel launch with timing events (no-ops
+// unless dh_timing_enable switched timing on for the handle).
+void dh_timing_begin(dh_handle_t h);
+void dh_timing_end(dh_handle_t h);
 
 // Release the batched plan / tuning table of a handle (batch.hip).
 void dh_batch_release(dh_handle_t h);

@@ -11,6 +11,9 @@
 #include <cstring>
 #include <new>
 
+#include <utility>
+#include <vector>
+
 #include "dh_common.h"
 
 static thread_local char g_last_error[1024] = "";
@@ -85,6 +88,7 @@ extern "C" int dh_destroy(dh_handle_t h) {
     (void)hipStreamSynchronize(h->stream);
     dh_batch_release(h);
     dh_tune_release(h);
+    dh_timing_enable(h, 0);
     for (int s = 0; s < DH_WS_COUNT; ++s)
         if (h->ws[s]) (void)hipFree(h->ws[s]);
     if (h->own_stream) (void)hipStreamDestroy(h->stream);
@@ -190,4 +194,79 @@ extern "C" int dh_convert_f32_f64(dh_handle_t h, int m, int n, const float* A, i
 extern "C" int dh_convert_f64_f32(dh_handle_t h, int m, int n, const double* A, int lda, float* B,
                                   int ldb) {
     return convert(h, "dh_convert_f64_f32", m, n, A, lda, B, ldb);
+}
+
+// ------------------------------------------------------------ kernel timing
+
+namespace {
+struct Timing {
+    std::vector<hipEvent_t> pool;                         // reusable events
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> open;  // recorded pairs
+    hipEvent_t pending = nullptr;                         // begin without end yet
+    hipEvent_t take() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+};
+}  // namespace
+
+void dh_timing_begin(dh_handle_t h) {
+    Timing* t = static_cast<Timing*>(h->timing);
+    if (!t) return;
+    t->pending = t->take();
+    if (t->pending) (void)hipEventRecord(t->pending, h->stream);
+}
+
+void dh_timing_end(dh_handle_t h) {
+    Timing* t = static_cast<Timing*>(h->timing);
+    if (!t || !t->pending) return;
+    hipEvent_t e = t->take();
+    if (!e) return;
+    (void)hipEventRecord(e, h->stream);
+    t->open.emplace_back(t->pending, e);
+    t->pending = nullptr;
+}
+
+extern "C" int dh_timing_enable(dh_handle_t h, int on) {
+    if (!h) return dh_fail(-1, "dh_timing_enable: NULL handle");
+    Timing* t = static_cast<Timing*>(h->timing);
+    if (on && !t) h->timing = new Timing();
+    if (!on && t) {
+        (void)hipStreamSynchronize(h->stream);
+        for (auto& pr : t->open) {
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+        for (hipEvent_t e : t->pool) (void)hipEventDestroy(e);
+        if (t->pending) (void)hipEventDestroy(t->pending);
+        delete t;
+        h->timing = nullptr;
+    }
+    return 0;
+}
+
+extern "C" int dh_timing_read(dh_handle_t h, double* ms, int* launches) {
+    if (!h) return dh_fail(-1, "dh_timing_read: NULL handle");
+    if (!ms || !launches) return dh_fail(-2, "dh_timing_read: NULL output");
+    *ms = 0.0;
+    *launches = 0;
+    Timing* t = static_cast<Timing*>(h->timing);
+    if (!t) return 0;
+    DH_CHECK_HIP(hipStreamSynchronize(h->stream));
+    for (auto& pr : t->open) {
+        float v = 0.f;
+        DH_CHECK_HIP(hipEventElapsedTime(&v, pr.first, pr.second));
+        *ms += v;
+        ++*launches;
+        t->pool.push_back(pr.first);
+        t->pool.push_back(pr.second);
+    }
+    t->open.clear();
+    return 0;
 }

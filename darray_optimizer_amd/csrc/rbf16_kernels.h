@@ -186,7 +186,14 @@ __device__ __forceinline__ void project_rows(
         }
         if (wave == WAVES - 1) dma4(bj + (size_t)t * 32 + (lane & 31), &sB[buf][0]);
     };
+    // Distance operands: all 2 x DK A-operand reads of a chunk are issued
+    // before its MFMAs.
     auto distance = [&](int buf, f32x4 (&s)[2][RB]) {
+        bf16x8 a[DK][2];
+#pragma unroll
+        for (int ks = 0; ks < DK; ++ks)
+#pragma unroll
+            for (int jb = 0; jb < 2; ++jb) a[ks][jb] = sX[buf][(16 * jb + li) * XROW + ks * 4 + g];
 #pragma unroll
         for (int jb = 0; jb < 2; ++jb)
 #pragma unroll
@@ -194,11 +201,10 @@ __device__ __forceinline__ void project_rows(
 #pragma unroll
         for (int ks = 0; ks < DK; ++ks)
 #pragma unroll
-            for (int jb = 0; jb < 2; ++jb) {
-                bf16x8 a = sX[buf][(16 * jb + li) * XROW + ks * 4 + g];
+            for (int jb = 0; jb < 2; ++jb)
 #pragma unroll
-                for (int ib = 0; ib < RB; ++ib) s[jb][ib] = mfma16<F16>(a, xb[ib][ks], s[jb][ib]);
-            }
+                for (int ib = 0; ib < RB; ++ib)
+                    s[jb][ib] = mfma16<F16>(a[ks][jb], xb[ib][ks], s[jb][ib]);
     };
     // K = clamp(exp2(c2 s + a_i + b_j), 0, 1) as the projection's A operand.
     auto epilogue = [&](int buf, const f32x4 (&s)[2][RB], bf16x8 (&kb)[RB]) {
@@ -214,10 +220,18 @@ __device__ __forceinline__ void project_rows(
                 }
             }
     };
-    auto project = [&](int buf, const bf16x8 (&kb)[RB]) {
+    // Projection B operands go through a ring of PF registers: the read of
+    // column block cb + PF is issued before the MFMAs of block cb.
+    constexpr int PF = NCB < 4 ? NCB : 4;
+    auto project_start = [&](int buf, bf16x8 (&ring)[PF]) {
+#pragma unroll
+        for (int p = 0; p < PF; ++p) ring[p] = sO[buf][p * 64 + lane];
+    };
+    auto project = [&](int buf, const bf16x8 (&kb)[RB], bf16x8 (&ring)[PF]) {
 #pragma unroll
         for (int cb = 0; cb < NCB; ++cb) {
-            bf16x8 bo = sO[buf][cb * 64 + lane];
+            const bf16x8 bo = ring[cb % PF];
+            if (cb + PF < NCB) ring[cb % PF] = sO[buf][(cb + PF) * 64 + lane];
 #pragma unroll
             for (int ib = 0; ib < RB; ++ib) acc[ib][cb] = mfma16<F16>(kb[ib], bo, acc[ib][cb]);
         }
@@ -234,22 +248,30 @@ __device__ __forceinline__ void project_rows(
     wait_vm();
     __syncthreads();
     f32x4 s[2][RB];
-    bf16x8 kb_prev[RB], kb[RB];
-    int buf = 0;  // buffer of chunk t
-    for (int t = 0; t < nchunks; ++t) {
+    bf16x8 kb_prev[RB], kb[RB], ring[PF];
+    // Step 0 (no projection yet), then steps 1 .. nchunks-1 as one basic block.
+    stage(min(1, last), 1);
+    distance(0, s);
+    epilogue(0, s, kb_prev);
+    wait_vm();
+    __syncthreads();
+    int buf = 1;  // buffer of chunk t
+    for (int t = 1; t < nchunks; ++t) {
         const int nbuf = (buf == NBUF - 1) ? 0 : buf + 1;   // chunk t+1
         const int pbuf = (buf == 0) ? NBUF - 1 : buf - 1;   // chunk t-1
         stage(min(t + 1, last), nbuf);
+        project_start(pbuf, ring);
         distance(buf, s);
         epilogue(buf, s, kb);
-        if (t > 0) project(pbuf, kb_prev);
+        project(pbuf, kb_prev, ring);
 #pragma unroll
         for (int ib = 0; ib < RB; ++ib) kb_prev[ib] = kb[ib];
         wait_vm();
         __syncthreads();
         buf = nbuf;
     }
-    project(buf == 0 ? NBUF - 1 : buf - 1, kb_prev);  // chunk last, after the final barrier
+    project_start(buf == 0 ? NBUF - 1 : buf - 1, ring);
+    project(buf == 0 ? NBUF - 1 : buf - 1, kb_prev, ring);  // last chunk, after the final barrier
 
 #pragma unroll
     for (int ib = 0; ib < RB; ++ib)
@@ -342,7 +364,9 @@ int launch_bf16(dh_handle_t h, int m, int groups, int nchunks, int l, int ncb_to
             dh_tune_store(h, key, sched);
         }
     }
+    dh_timing_begin(h);
     run(sched);
+    dh_timing_end(h);
     DH_CHECK_LAUNCH();
     return 0;
 }

@@ -228,8 +228,10 @@ static void launch_project_f64(dh_handle_t h, dim3 grid, int m, int n, int d, in
                                const double* ni, const double* nj, const double* yi,
                                const double* yj, double gamma, const double* O, int ldo,
                                double beta, double* Y, int ldy) {
+    dh_timing_begin(h);
     rbf_project_f64_kernel<DK, NCB><<<grid, 256, 0, h->stream>>>(
         m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, O, ldo, beta, Y, ldy);
+    dh_timing_end(h);
 }
 
 template <int DK>

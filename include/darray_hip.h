@@ -51,6 +51,15 @@ const char* dh_last_error_string(void);
 /* Bytes of device workspace currently held by the handle. */
 size_t dh_workspace_bytes(dh_handle_t h);
 
+/* Kernel timing for measurements (no reference counterpart; DArray times
+ * phases with a host millisecond clock, utils.h:436-456).  With on != 0 the
+ * handle records HIP events on its stream around every fused projection
+ * kernel it launches (dh_rbf_project_*, and inside dh_lra); dh_timing_read
+ * synchronises, returns the summed kernel milliseconds and the number of
+ * launches since the previous read, and resets them. */
+int dh_timing_enable(dh_handle_t h, int on);
+int dh_timing_read(dh_handle_t h, double* ms, int* launches);
+
 /* ------------------------------------------------- device memory helpers */
 /* For hosts that include no HIP header (the DArray C++ mirror in darray_hip/).
  * The reference allocates with cudaMallocManaged per call (dkernel.cu:135-147)

@@ -153,3 +153,22 @@ def test_lra_mixed_error_close_to_fp64(dev, labels):
     assert abs(errs[dh.DH_PREC_F64] - np.sqrt(eo / ko)) <= 1e-6 * np.sqrt(eo / ko) + 1e-12
     assert errs[dh.DH_PREC_BF16] <= 1.5 * errs[dh.DH_PREC_F64] + 3e-3
     assert errs[dh.DH_PREC_F16] <= 1.2 * errs[dh.DH_PREC_F64] + 5e-4
+
+
+def test_kernel_timing_counts_projection_launches(dev):
+    import torch
+    n, d, l = 1024, 64, 40
+    X = dh.fill_mixture(9, d, n, 4, 0.3)
+    O = np.asfortranarray(dh.fill_normal(10, n, l))
+    Xd, Od = dev_pts(X), dh.to_dev(O)
+    Y = torch.empty((l, n), dtype=torch.float64, device="cuda")
+    dev.rbf_project_bf16(Xd, Xd, Od, 1.0 / d, Y)  # schedule settled before timing
+    dev.timing(True)
+    dev.timing_read()
+    for _ in range(3):
+        dev.rbf_project_bf16(Xd, Xd, Od, 1.0 / d, Y)
+    dev.rbf_project_f64(Xd, Xd, Od, 1.0 / d, Y)
+    ms, launches = dev.timing_read()
+    dev.timing(False)
+    assert launches == 4 and ms > 0.0
+    assert dev.timing_read() == (0.0, 0)
