@@ -7,7 +7,8 @@ resident in HBM.  GFLOP/s counts F_MFMA + F_orth:
     F_MFMA = (q+2) * (2 n^2 d + 2 n^2 l),  F_orth = (q+1) * (6 n l^2 + 2/3 l^3)
 Besides the headline (BF16 K-passes) the line carries the FP64 solve, the
 FP16 K-passes, the 16384 core solve, the batched workload (96 problems,
-one graph) and the CPU baseline.
+one graph), the largest workload (n = 262144, d = 256, k = 512, FP8
+distance) and the CPU baseline.
 Prints ONE JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
 """
 from __future__ import annotations
@@ -150,6 +151,54 @@ def batched_workload(h, count, seed, with_ref):
     return out
 
 
+def largest_workload(h, n, d, k, p, q, with_ref):
+    """n points of a 32-centre mixture in [-1,1]^d (sigma 0.2), gamma = 0.25/d,
+    rank k + p, q power iterations: FP8 e4m3 distance + BF16 projection, K
+    never stored; errors against the FP64 path on the same inputs."""
+    import torch
+    import darray_optimizer_amd as dh
+    l = k + p
+    gamma = 0.25 / d
+    X = torch.from_numpy(dh.fill_mixture(4, d, n, 32, 0.2)).cuda()
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+    Om = torch.randn((l, n), dtype=torch.float64, device="cuda", generator=gen)
+    Q = torch.empty((l, n), dtype=torch.float64, device="cuda")
+    W = torch.empty((l, n), dtype=torch.float64, device="cuda")
+    fm, fo = lra_flops(n, d, l, q)
+    assert h.lra(X, Om, gamma, q, Q, W, prec=dh.DH_PREC_FP8) == 0  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    assert h.lra(X, Om, gamma, q, Q, W, prec=dh.DH_PREC_FP8) == 0
+    torch.cuda.synchronize()
+    s = time.perf_counter() - t0
+    out = {"n": n, "d": d, "k": k, "p": p, "q": q, "l": l, "gamma": gamma, "centres": 32,
+           "sigma": 0.2, "precision": "fp8 e4m3 distance (per-row scales), bf16 projection, "
+                                      "fp64 CholeskyQR2", "s_per_solve": s,
+           "gflops": (fm + fo) / s / 1e9}
+    if with_ref:
+        e2, k2 = h.lra_error(X, gamma, Q, W)
+        out["rel_err"] = float(np.sqrt(max(e2, 0.0) / k2))
+        # The same solve with a BF16 distance: separates the FP8 rounding from the BF16 one.
+        t0 = time.perf_counter()
+        assert h.lra(X, Om, gamma, q, Q, W, prec=dh.DH_PREC_BF16) == 0
+        torch.cuda.synchronize()
+        out["bf16_distance_s_per_solve"] = time.perf_counter() - t0
+        b2, _ = h.lra_error(X, gamma, Q, W)
+        out["bf16_distance_rel_err"] = float(np.sqrt(max(b2, 0.0) / k2))
+        t0 = time.perf_counter()
+        assert h.lra(X, Om, gamma, q, Q, W, prec=dh.DH_PREC_F64) == 0
+        torch.cuda.synchronize()
+        out["fp64_s_per_solve"] = time.perf_counter() - t0
+        f2, _ = h.lra_error(X, gamma, Q, W)
+        out["fp64_rel_err"] = float(np.sqrt(max(f2, 0.0) / k2))
+        out["err_ratio_vs_fp64"] = out["rel_err"] / out["fp64_rel_err"]
+        out["speedup_vs_fp64"] = out["fp64_s_per_solve"] / s
+    del X, Om, Q, W
+    torch.cuda.empty_cache()
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +216,8 @@ def main(argv=None) -> int:
     ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--batched", type=int, default=96, help="batched problems (0: skip)")
+    ap.add_argument("--largest", type=int, default=262144,
+                    help="points of the FP8-distance workload (0: skip)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
@@ -260,6 +311,8 @@ def main(argv=None) -> int:
     f16["speedup_vs_fp64"] = fp64["s_per_solve"] / f16["s_per_solve"]
     if a.batched > 0:
         line["batched"] = batched_workload(h, a.batched, 3, not a.no_error)
+    if a.largest > 0:
+        line["largest_fp8"] = largest_workload(h, a.largest, 256, 512, 32, 1, not a.no_error)
     if a.core > 0:
         line["core_solve"] = core_solve(h, Xd, gamma, a.core, a.nrhs, stream)
     if not a.no_cpu:

@@ -14,9 +14,9 @@ import ctypes as C
 import numpy as np
 
 from . import _capi
-from ._capi import DH_PREC_BF16, DH_PREC_F16, DH_PREC_F64, DHError, call, lib
+from ._capi import DH_PREC_BF16, DH_PREC_F16, DH_PREC_F64, DH_PREC_FP8, DHError, call, lib
 
-__all__ = ["Handle", "DHError", "DH_PREC_F64", "DH_PREC_BF16", "DH_PREC_F16", "lib", "fill_uniform",
+__all__ = ["Handle", "DHError", "DH_PREC_F64", "DH_PREC_BF16", "DH_PREC_F16", "DH_PREC_FP8", "lib", "fill_uniform",
            "fill_normal", "fill_mixture", "mt19937_raw", "to_dev", "from_dev"]
 
 
@@ -120,6 +120,15 @@ class Handle:
         n = Xj.shape[0]
         l = O.shape[0]
         call("dh_rbf_project_bf16", self.h, m, n, d, l, _ptr(Xi), _ld(Xi), _ptr(Xj), _ld(Xj),
+             _ptr(yi), _ptr(yj), float(gamma), _ptr(O), _ld(O), _ptr(Y), _ld(Y))
+        return Y
+
+    def rbf_project_fp8(self, Xi, Xj, O, gamma, Y, yi=None, yj=None):
+        """FP8 e4m3 distance, BF16 projection (FP64 in/out)."""
+        m, d = Xi.shape
+        n = Xj.shape[0]
+        l = O.shape[0]
+        call("dh_rbf_project_fp8", self.h, m, n, d, l, _ptr(Xi), _ld(Xi), _ptr(Xj), _ld(Xj),
              _ptr(yi), _ptr(yj), float(gamma), _ptr(O), _ld(O), _ptr(Y), _ld(Y))
         return Y
 

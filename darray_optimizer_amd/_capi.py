@@ -44,6 +44,8 @@ SIGNATURES: dict[str, tuple] = {
                                  _i]),
     "dh_rbf_project_f16": (_i, [_p, _i, _i, _i, _i, _p, _i, _p, _i, _p, _p, _d, _p, _i, _p,
                                 _i]),
+    "dh_rbf_project_fp8": (_i, [_p, _i, _i, _i, _i, _p, _i, _p, _i, _p, _p, _d, _p, _i, _p,
+                                _i]),
     "dh_gemm_f64": (_i, [_p, _c, _c, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p, _i]),
     "dh_gemm_f32": (_i, [_p, _c, _c, _i, _i, _i, _f, _p, _i, _p, _i, _f, _p, _i]),
     "dh_syrk_ut_f64": (_i, [_p, _i, _i, _d, _p, _i, _d, _p, _i]),
@@ -69,6 +71,7 @@ SIGNATURES: dict[str, tuple] = {
 DH_PREC_F64 = 0
 DH_PREC_BF16 = 1
 DH_PREC_F16 = 2
+DH_PREC_FP8 = 3
 DH_ERR_HIP = -1000
 DH_ERR_ALLOC = -1001
 DH_ERR_UNSUPPORTED = -1002

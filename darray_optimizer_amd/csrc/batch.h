@@ -31,7 +31,7 @@ struct DhProjJob {
     int m, nchunks, l, ncb_pack;
     const void* xb;
     const float* a;
-    float c2, inv_c2;
+    float c2;
     const void* Op;
     const double* y;
     double* Y;

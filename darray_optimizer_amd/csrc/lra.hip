@@ -18,6 +18,9 @@ int dh_lra_bf16_passes(dh_handle_t h, int n, int d, int l, int q, const double* 
 int dh_lra_f16_passes(dh_handle_t h, int n, int d, int l, int q, const double* X, int ldx,
                       const double* y, double gamma, const double* Omega, int ldo, double* Q,
                       int ldq, double* W, int ldw);
+int dh_lra_fp8_passes(dh_handle_t h, int n, int d, int l, int q, const double* X, int ldx,
+                      const double* y, double gamma, const double* Omega, int ldo, double* Q,
+                      int ldq, double* W, int ldw);
 
 // dst (n x n, ld n) = triu(src); strict lower set to zero.
 __global__ __launch_bounds__(256) void triu_copy_kernel(int n, const double* __restrict__ src,
@@ -296,8 +299,7 @@ extern "C" int dh_lra(dh_handle_t h, int prec, int n, int d, int l, int q, const
                       double* Q, int ldq, double* W, int ldw) {
     const char* name = "dh_lra";
     if (!h) return dh_fail(-1, "%s: NULL handle", name);
-    if (prec != DH_PREC_F64 && prec != DH_PREC_BF16 && prec != DH_PREC_F16)
-        return dh_fail(-2, "%s: bad prec %d", name, prec);
+    if (prec < DH_PREC_F64 || prec > DH_PREC_FP8) return dh_fail(-2, "%s: bad prec %d", name, prec);
     if (n < 0) return dh_fail(-3, "%s: n < 0", name);
     if (d < 1) return dh_fail(-4, "%s: d < 1", name);
     if (l < 0 || l > n) return dh_fail(-5, "%s: need 0 <= l <= n", name);
@@ -312,8 +314,10 @@ extern "C" int dh_lra(dh_handle_t h, int prec, int n, int d, int l, int q, const
         st = lra_f64(h, n, d, l, q, X, ldx, y, gamma, Omega, ldo, Q, ldq, W, ldw);
     else if (prec == DH_PREC_BF16)
         st = dh_lra_bf16_passes(h, n, d, l, q, X, ldx, y, gamma, Omega, ldo, Q, ldq, W, ldw);
-    else
+    else if (prec == DH_PREC_F16)
         st = dh_lra_f16_passes(h, n, d, l, q, X, ldx, y, gamma, Omega, ldo, Q, ldq, W, ldw);
+    else
+        st = dh_lra_fp8_passes(h, n, d, l, q, X, ldx, y, gamma, Omega, ldo, Q, ldq, W, ldw);
     if (st) return st;
     DH_CHECK_HIP(hipStreamSynchronize(h->stream));
     return 0;

@@ -22,7 +22,7 @@ __global__ __launch_bounds__(WAVES * 64, 1) void rbf_project_bf16_persistent(
         const DhProjJob& j = jobs[w.job];
         project_rows<DK, NCB, WAVES, false, RB>(w.rblk, w.cg, j.m, j.nchunks, j.l, j.ncb_pack,
                                                 (const bf16x8*)j.xb, (const bf16x8*)j.xb, j.a, j.a,
-                                                j.c2, j.inv_c2, (const bf16x8*)j.Op, j.y, j.Y,
+                                                nullptr, nullptr, j.c2, (const bf16x8*)j.Op, j.y, j.Y,
                                                 j.ldy);
     }
 }

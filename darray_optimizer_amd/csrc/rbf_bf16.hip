@@ -87,6 +87,51 @@ __global__ __launch_bounds__(256) void maxabs_centred_kernel(int n, int d,
 
 __global__ void zero_word_kernel(unsigned* w) { *w = 0u; }
 
+// One wave per point row, features 4 lane .. 4 lane + 3 (dpad8 <= 256):
+// q = e4m3((X - mean) * scale_i) with scale_i = 2^(8 - e) for the row's
+// max |x - mean| = f 2^e (f in [1/2, 1)), so its largest magnitude lies in
+// [128, 256) (e4m3 reaches 448); a[i] = neg_gl2e |q / scale_i|^2 from the
+// dequantised values in FP32; sinv[i] = 1 / scale_i.  Rows >= n are zero.
+__global__ __launch_bounds__(256) void convert_rows_fp8_kernel(int n, int nrows, int d, int dpad8,
+                                                               const double* __restrict__ X,
+                                                               int ldx,
+                                                               const double* __restrict__ mean,
+                                                               float neg_gl2e,
+                                                               unsigned* __restrict__ Xb8,
+                                                               float* __restrict__ a,
+                                                               float* __restrict__ sinv) {
+    const int row = (blockIdx.x * 256 + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    if (row >= nrows) return;
+    float v[4], mx = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int f = 4 * lane + k;
+        v[k] = (row < n && f < d) ? (float)(X[f + (size_t)row * ldx] - mean[f]) : 0.f;
+        mx = fmaxf(mx, fabsf(v[k]));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    float scale = 1.f, inv = 0.f;
+    if (mx > 0.f) {
+        int e;
+        frexpf(mx, &e);
+        scale = ldexpf(1.f, 8 - e);
+        inv = ldexpf(1.f, e - 8);
+    }
+    int w = __builtin_amdgcn_cvt_pk_fp8_f32(v[0] * scale, v[1] * scale, 0, false);
+    w = __builtin_amdgcn_cvt_pk_fp8_f32(v[2] * scale, v[3] * scale, w, true);
+    const float q0 = __builtin_amdgcn_cvt_f32_fp8(w, 0) * inv, q1 = __builtin_amdgcn_cvt_f32_fp8(w, 1) * inv;
+    const float q2 = __builtin_amdgcn_cvt_f32_fp8(w, 2) * inv, q3 = __builtin_amdgcn_cvt_f32_fp8(w, 3) * inv;
+    float nrm = (q0 * q0 + q1 * q1) + (q2 * q2 + q3 * q3);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) nrm += __shfl_xor(nrm, o);
+    if (4 * lane < dpad8) Xb8[(size_t)row * (dpad8 / 4) + lane] = (unsigned)w;
+    if (lane == 0) {
+        a[row] = (row < n) ? neg_gl2e * nrm : 0.f;
+        sinv[row] = (row < n) ? inv : 0.f;
+    }
+}
+
 // Packed operand of the projection MFMA.  For chunk t (32 points), column
 // block cb (16 columns) and lane L = 16 g + li, the 8 BF16 values are
 //   Op[t][cb][L][e] = y[j] * O[j][16 cb + li],  j = 32 t + 16 (e / 4) + 4 g + e % 4,
@@ -323,6 +368,97 @@ int rbf_project_16(const char* name, dh_handle_t h, int m, int n, int d, int l, 
 
 }  // namespace
 
+namespace {
+
+// FP8 copy of one point set inside a workspace (one byte per feature).
+struct Fp8Points {
+    unsigned* xb = nullptr;  // nrows_pad x dpad8 bytes
+    float* a = nullptr;      // nrows_pad
+    float* sinv = nullptr;   // nrows_pad
+    int nrows_pad = 0;
+};
+
+int dpad8_of(int d) { return d <= 128 ? 128 : 256; }
+
+// Carve and fill the FP8 copies of one or two point sets (centred by the
+// mean of the column set Xj, rows scaled separately).
+int prepare_points_fp8(dh_handle_t h, int m, int n, int d, const double* Xi, int ldxi,
+                       const double* Xj, int ldxj, double gamma, bool same, Fp8Points& pi,
+                       Fp8Points& pj) {
+    const int dp = dpad8_of(d);
+    const int mpad = dh_cdiv(m, 32) * 32, npad = dh_cdiv(n, 32) * 32;
+    auto bytes_of = [&](int rows) { return ((size_t)rows * dp + 8 * (size_t)rows + 511) & ~size_t(255); };
+    const size_t bj = bytes_of(npad), bi = same ? 0 : bytes_of(mpad);
+    char* base = (char*)dh_ws(h, DH_WS_LOWP_X, bj + bi + 512);
+    if (!base) return DH_ERR_ALLOC;
+    double* mean = (double*)dh_ws(h, DH_WS_NORMS, sizeof(double) * (size_t)d + 64);
+    if (!mean) return DH_ERR_ALLOC;
+    auto carve = [&](char* p, int rows, Fp8Points& out) {
+        out.xb = (unsigned*)p;
+        out.a = (float*)(p + (size_t)rows * dp);
+        out.sinv = out.a + rows;
+        out.nrows_pad = rows;
+    };
+    carve(base, npad, pj);
+    if (same)
+        pi = pj;
+    else
+        carve(base + bj, mpad, pi);
+    DH_CHECK_STATUS(point_mean(h, n, d, Xj, ldxj, mean));
+    const float neg = (float)(-gamma) * kLog2e;
+    convert_rows_fp8_kernel<<<dh_cdiv(npad, 4), 256, 0, h->stream>>>(n, npad, d, dp, Xj, ldxj,
+                                                                     mean, neg, pj.xb, pj.a,
+                                                                     pj.sinv);
+    DH_CHECK_LAUNCH();
+    if (!same) {
+        convert_rows_fp8_kernel<<<dh_cdiv(mpad, 4), 256, 0, h->stream>>>(
+            m, mpad, d, dp, Xi, ldxi, mean, neg, pi.xb, pi.a, pi.sinv);
+        DH_CHECK_LAUNCH();
+    }
+    return 0;
+}
+
+int project_fp8(dh_handle_t h, int m, int n, int d, int l, const Fp8Points& pi,
+                const Fp8Points& pj, float c2, const bf16x8* Op, const double* yi, double* Y,
+                int ldy) {
+    if (m == 0 || l == 0) return 0;
+    Bf16Cols cols = bf16_cols(l);
+    return dh_project_fp8(h, dpad8_of(d), cols.ncb, cols.groups, m, dh_cdiv(n, 32), l,
+                          cols.ncb_pack, (const bf16x8*)pi.xb, (const bf16x8*)pj.xb, pi.a, pj.a,
+                          pi.sinv, pj.sinv, c2, Op, yi, Y, ldy);
+}
+
+}  // namespace
+
+extern "C" int dh_rbf_project_fp8(dh_handle_t h, int m, int n, int d, int l, const double* Xi,
+                                  int ldxi, const double* Xj, int ldxj, const double* yi,
+                                  const double* yj, double gamma, const double* O, int ldo,
+                                  double* Y, int ldy) {
+    const char* name = "dh_rbf_project_fp8";
+    if (!h) return dh_fail(-1, "%s: NULL handle", name);
+    if (m < 0) return dh_fail(-2, "%s: m < 0", name);
+    if (n < 0) return dh_fail(-3, "%s: n < 0", name);
+    if (d < 1 || d > 256) return dh_fail(-4, "%s: need 1 <= d <= 256", name);
+    if (l < 0) return dh_fail(-5, "%s: l < 0", name);
+    if (ldxi < d) return dh_fail(-7, "%s: ldxi < d", name);
+    if (ldxj < d) return dh_fail(-9, "%s: ldxj < d", name);
+    if (ldo < max(1, n)) return dh_fail(-14, "%s: ldo < n", name);
+    if (ldy < max(1, m)) return dh_fail(-16, "%s: ldy < m", name);
+    if (m == 0 || l == 0) return 0;
+    if (n == 0) {
+        DH_CHECK_HIP(hipMemset2DAsync(Y, sizeof(double) * (size_t)ldy, 0, sizeof(double) * (size_t)m,
+                                      l, h->stream));
+        return 0;
+    }
+    Fp8Points pi, pj;
+    const bool same = (Xi == Xj && m == n && ldxi == ldxj);
+    DH_CHECK_STATUS(prepare_points_fp8(h, m, n, d, Xi, ldxi, Xj, ldxj, gamma, same, pi, pj));
+    bf16x8* Op = (bf16x8*)dh_ws(h, DH_WS_LOWP_O, packed_omega_bytes(n, l));
+    if (!Op) return DH_ERR_ALLOC;
+    DH_CHECK_STATUS(pack_omega<false>(h, n, l, O, ldo, yj, Op));
+    return project_fp8(h, m, n, d, l, pi, pj, (float)(2.0 * gamma) * kLog2e, Op, yi, Y, ldy);
+}
+
 extern "C" int dh_rbf_project_bf16(dh_handle_t h, int m, int n, int d, int l, const double* Xi,
                                    int ldxi, const double* Xj, int ldxj, const double* yi,
                                    const double* yj, double gamma, const double* O, int ldo,
@@ -372,6 +508,31 @@ int dh_lra_bf16_passes(dh_handle_t h, int n, int d, int l, int q, const double* 
                        const double* y, double gamma, const double* Omega, int ldo, double* Q,
                        int ldq, double* W, int ldw) {
     return lra_16_passes<false>(h, n, d, l, q, X, ldx, y, gamma, Omega, ldo, Q, ldq, W, ldw);
+}
+
+// FP8-distance passes: FP8 e4m3 distance, BF16 projection, FP64 CholeskyQR2.
+int dh_lra_fp8_passes(dh_handle_t h, int n, int d, int l, int q, const double* X, int ldx,
+                      const double* y, double gamma, const double* Omega, int ldo, double* Q,
+                      int ldq, double* W, int ldw) {
+    if (d > 256) return dh_fail(DH_ERR_UNSUPPORTED, "dh_lra: FP8 passes need d <= 256");
+    Fp8Points p, p2;
+    DH_CHECK_STATUS(prepare_points_fp8(h, n, n, d, X, ldx, X, ldx, gamma, true, p, p2));
+    bf16x8* Op = (bf16x8*)dh_ws(h, DH_WS_LOWP_O, packed_omega_bytes(n, l));
+    if (!Op) return DH_ERR_ALLOC;
+    const float c2 = (float)(2.0 * gamma) * kLog2e;
+    DH_CHECK_STATUS(pack_omega<false>(h, n, l, Omega, ldo, y, Op));
+    DH_CHECK_STATUS(project_fp8(h, n, n, d, l, p, p, c2, Op, y, Q, ldq));
+    DH_CHECK_STATUS(dh_range_orth(h, n, l, Q, ldq));
+    for (int it = 0; it < q; ++it) {
+        DH_CHECK_STATUS(pack_omega<false>(h, n, l, Q, ldq, y, Op));
+        DH_CHECK_STATUS(project_fp8(h, n, n, d, l, p, p, c2, Op, y, W, ldw));
+        DH_CHECK_STATUS(dh_range_orth(h, n, l, W, ldw));
+        DH_CHECK_HIP(hipMemcpy2DAsync(Q, sizeof(double) * (size_t)ldq, W,
+                                      sizeof(double) * (size_t)ldw, sizeof(double) * (size_t)n, l,
+                                      hipMemcpyDeviceToDevice, h->stream));
+    }
+    DH_CHECK_STATUS(pack_omega<false>(h, n, l, Q, ldq, y, Op));
+    return project_fp8(h, n, n, d, l, p, p, c2, Op, y, W, ldw);
 }
 
 int dh_lra_f16_passes(dh_handle_t h, int n, int d, int l, int q, const double* X, int ldx,

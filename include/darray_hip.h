@@ -112,6 +112,15 @@ int dh_rbf_project_f16(dh_handle_t h, int m, int n, int d, int l, const double* 
                        const double* Xj, int ldxj, const double* yi, const double* yj,
                        double gamma, const double* O, int ldo, double* Y, int ldy);
 
+/* As dh_rbf_project_bf16 with the distance in FP8 e4m3 (points centred by the
+ * mean of Xj, each point scaled by a power of two so that its largest |x| lies
+ * in [128, 256), one byte per feature; FP32 norms of the dequantised points;
+ * 16x16x128 FP8 MFMA with FP32 accumulation) and the projection in BF16.
+ * d <= 256. */
+int dh_rbf_project_fp8(dh_handle_t h, int m, int n, int d, int l, const double* Xi, int ldxi,
+                       const double* Xj, int ldxj, const double* yi, const double* yj,
+                       double gamma, const double* O, int ldo, double* Y, int ldy);
+
 /* Mixed-precision fused Y[m x l] = K(Xi,Xj) * O (Y overwritten): points centred by
  * the mean of Xj and rounded to BF16, FP32 squared norms of the rounded points,
  * BF16 MFMA distance with FP32 accumulation, K = exp2(..) on v_exp_f32 rounded
@@ -197,6 +206,7 @@ int dh_cholqr2_f64(dh_handle_t h, int m, int n, double* Y, int ldy, double* R, i
 #define DH_PREC_F64 0  /* FP64 MFMA distance + projection, FP64 exp */
 #define DH_PREC_BF16 1 /* BF16 MFMA distance + projection, FP32 accumulate and exp */
 #define DH_PREC_F16 2  /* as DH_PREC_BF16 with FP16 operands (dh_rbf_project_f16) */
+#define DH_PREC_FP8 3  /* FP8 e4m3 distance, BF16 projection (dh_rbf_project_fp8) */
 
 /* Randomised range finder on the RBF kernel of one point set (DArray::LRA,
  * dkernel.h:12-46, and the orthogonalisation of testsvm.cpp:102-108, fused):
@@ -239,7 +249,7 @@ typedef struct dh_lra_problem {
  * same.  Tile schedules come from the autotuner.  A problem whose
  * CholeskyQR breaks down is redone alone by dh_lra (shifted CholeskyQR3,
  * completion of rank-deficient bases).
- * DH_PREC_F64 and DH_PREC_F16 run dh_lra per problem.  status (host, may be NULL) receives
+ * DH_PREC_F64, DH_PREC_F16 and DH_PREC_FP8 run dh_lra per problem.  status (host, may be NULL) receives
  * each problem's dh_lra status; returns 0 or the first non-zero status.
  * All problems must share q.  Synchronises. */
 int dh_lra_batched(dh_handle_t h, int prec, int count, const dh_lra_problem* problems,

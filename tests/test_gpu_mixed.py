@@ -172,3 +172,78 @@ def test_kernel_timing_counts_projection_launches(dev):
     dev.timing(False)
     assert launches == 4 and ms > 0.0
     assert dev.timing_read() == (0.0, 0)
+
+
+def e4m3(v):
+    """Round to the nearest OCP FP8 e4m3 value (ties to even); |v| < 448 assumed."""
+    v = np.asarray(v, dtype=np.float64)
+    a = np.abs(v)
+    _, e = np.frexp(a)
+    E = np.maximum(e - 1, -6)            # exponent of the leading bit; subnormals below 2^-6
+    quantum = np.ldexp(1.0, E - 3)
+    return np.sign(v) * np.round(a / quantum) * quantum
+
+
+def emulate_fp8(Xi, Xj, O, gamma, yi=None, yj=None):
+    """FP8 distance (per-row power-of-two scales) and BF16 K / O, as the device computes."""
+    mu = Xj.mean(axis=0)
+
+    def quant(X):
+        C = np.asarray(X - mu, dtype=np.float32).astype(np.float64)
+        mx = np.abs(C).max(axis=1)
+        _, e = np.frexp(mx.astype(np.float32))
+        sc = np.where(mx > 0, np.ldexp(1.0, 8 - e), 1.0)
+        return e4m3(C * sc[:, None]) / sc[:, None]
+
+    a, b = quant(Xi), quant(Xj)
+    na, nb = (a * a).sum(1), (b * b).sum(1)
+    d2 = np.maximum(na[:, None] + nb[None, :] - 2 * a @ b.T, 0)
+    K = bf16(np.exp(-gamma * d2))
+    Ow = O * (yj[:, None] if yj is not None else 1.0)
+    Y = K @ bf16(Ow)
+    return Y * yi[:, None] if yi is not None else Y
+
+
+def test_e4m3_rounding_known_values():
+    # Values the device conversion was seen to produce (OCP e4m3): 1.0625 -> 1, 1.1 -> 1.125,
+    # 448 stays, 0.001 -> 2^-9 (subnormal step), 0.0703125 exact.
+    got = e4m3([1.0625, 1.1, 448.0, 0.001, 0.0703125, -3.0, 240.0])
+    assert list(got) == [1.0, 1.125, 448.0, 2.0**-9, 0.0703125, -3.0, 240.0]
+
+
+@pytest.mark.parametrize("m,n,d,l", [(64, 64, 13, 16), (300, 257, 64, 138), (200, 333, 128, 40),
+                                     (513, 1000, 130, 272), (96, 500, 256, 300)])
+def test_rbf_project_fp8(dev, m, n, d, l):
+    import torch
+    rng = np.random.default_rng(11 * m + n + d + l)
+    Xi, Xj = rng.uniform(-1, 1, (m, d)), rng.uniform(-1, 1, (n, d))
+    yi, yj = rng.choice([-1.0, 1.0], m), rng.choice([-1.0, 1.0], n)
+    O = rng.standard_normal((n, l))
+    gamma = 1.0 / d
+    Y = torch.empty((l, m), dtype=torch.float64, device="cuda")
+    dev.rbf_project_fp8(dev_pts(Xi), dev_pts(Xj), dh.to_dev(O), gamma, Y, dev_vec(yi), dev_vec(yj))
+    Yg = dh.from_dev(Y)
+    e_emu = rel(Yg, emulate_fp8(Xi, Xj, O, gamma, yi, yj))
+    e_orc = rel(Yg, orc.rbf_project(Xi, Xj, O, gamma, yi, yj))
+    print(f"fp8 m={m} n={n} d={d} l={l}: vs emulation {e_emu:.2e}, vs FP64 oracle {e_orc:.2e}")
+    assert e_emu < 5e-4
+    assert e_orc < 3e-2
+
+
+def test_lra_fp8_error_close_to_fp64(dev):
+    """C4-style data (mixture, gamma = 0.25/d, d = 256) at reduced n: the FP8-distance
+    range finder stays within a small factor of the FP64 approximation error."""
+    import torch
+    n, d, l, q = 4096, 256, 80, 1
+    X = dh.fill_mixture(21, d, n, 32, 0.2)
+    Om = dh.fill_normal(22, n, l)
+    Xd, Omd = dev_pts(X), dh.to_dev(Om)
+    errs = {}
+    for prec in (dh.DH_PREC_F64, dh.DH_PREC_FP8):
+        Q = torch.empty((l, n), dtype=torch.float64, device="cuda")
+        W = torch.empty((l, n), dtype=torch.float64, device="cuda")
+        assert dev.lra(Xd, Omd, 0.25 / d, q, Q, W, prec=prec) == 0
+        e2, k2 = dev.lra_error(Xd, 0.25 / d, Q, W)
+        errs[prec] = np.sqrt(e2 / k2)
+    print(f"LRA errors: fp64 {errs[dh.DH_PREC_F64]:.4e}  fp8 {errs[dh.DH_PREC_FP8]:.4e}")
+    assert errs[dh.DH_PREC_FP8] <= 1.5 * errs[dh.DH_PREC_F64] + 2e-3
