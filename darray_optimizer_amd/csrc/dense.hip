@@ -1,5 +1,5 @@
-// Dense building blocks on gfx950 MFMA: GEMM / SYRK (f32 and f64), small
-// in-LDS Cholesky and triangular-solve kernels, and the blocked drivers of
+// Dense building blocks on gfx950 MFMA: GEMM / SYRK (f32 and f64), 64 x 64
+// diagonal-block Cholesky / inverse kernels, and the blocked drivers of
 // the Cholesky and TRSM that DArray runs through BLAS/LAPACK:
 //   multiply_matrix / matrix_multiply  (cblas_like.h:39-60, blas_like.h:62-93)
 //   symmetric_update -> ssyrk_         (cblas_like.h:61-69)
@@ -174,203 +174,230 @@ template int dh_gemm_impl<float>(dh_handle_t, bool, bool, int, int, int, float, 
                                  int, const float*, int, float, float*, int, int);
 
 // ------------------------------------------------------ small triangles
-//
-// A triangle of order nb <= 64 lives in LDS; one thread owns one right-hand
-// side (a column for side L, a row for side R) in registers.
 
 constexpr int kSmall = 64;
 
-template <typename T>
-__device__ __forceinline__ void load_upper_lds(T (*s)[kSmall + 1], int nb, const T* A, int lda) {
-    for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
-        int i = e % nb, j = e / nb;
-        s[i][j] = (i <= j) ? A[i + (size_t)j * lda] : T(0);
-    }
-}
+// Diagonal blocks are stored as Linv = U_bb^{-T} (lower triangular),
+// column-major with ld 64, one 4096-element block per 64 rows of the
+// triangle; blocks of order nb < 64 are padded with the identity.  With it
+// the three solves of a diagonal block are products:
+//   U^{-T} B = Linv B,   U^{-1} B = Linv^T B,   B U^{-1} = B Linv^T.
 
-// B := U^{-T} B  (forward substitution), B nb x ncols.
+// Cholesky of the nb <= 64 diagonal block at A (upper triangle, in place)
+// together with Linv.  Symmetric Gaussian elimination on [A | I]: step k
+// subtracts m_i = A(k,i) / A(k,k) times row k from every row i > k, so the
+// left part ends as D Ut (Ut unit upper, A = Ut^T D Ut) and the right part
+// as Ut^{-T}; then U = D^{1/2} Ut and U^{-T} = D^{-1/2} Ut^{-T}.  Both parts
+// share one 64 x 64 register grid: thread t holds row i = t % 64 at columns
+// j = t / 64 + 4 q (q < 16), the left part where j >= i and the strictly
+// lower right part where j < i.  Row k is broadcast through LDS (double
+// buffered: one barrier per step).  A non-positive (or NaN) pivot at local
+// step k stores base + k + 1 in *info (first failure wins) and leaves A as
+// it was.
 template <typename T>
-__global__ __launch_bounds__(256) void trsm_lut_small_kernel(int nb, int ncols,
-                                                              const T* __restrict__ A, int lda,
-                                                              T* __restrict__ B, int ldb,
-                                                              int unit,
-                                                              const int* __restrict__ stop) {
-    __shared__ T s[kSmall][kSmall + 1];
-    if (stop && *stop) return;
-    load_upper_lds<T>(s, nb, A, lda);
-    __syncthreads();
-    int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= ncols) return;
-    T* b = B + (size_t)col * ldb;
-    T x[kSmall];
+__global__ __launch_bounds__(256) void potrf_inv_small_kernel(int nb, T* __restrict__ A, int lda,
+                                                               T* __restrict__ Linv,
+                                                               int* __restrict__ info, int base) {
+    __shared__ T rows[2][kSmall];
+    __shared__ T diag[kSmall];
+    if (*info != 0) return;
+    const int t = threadIdx.x, i = t & (kSmall - 1), w = t >> 6;
+    T a[16];
 #pragma unroll
-    for (int i = 0; i < kSmall; ++i) x[i] = (i < nb) ? b[i] : T(0);
+    for (int q = 0; q < 16; ++q) {
+        const int j = w + 4 * q;
+        a[q] = (i < nb && j < nb) ? (i <= j ? A[i + (size_t)j * lda] : T(0))
+                                  : (i == j ? T(1) : T(0));
+    }
+    int fail = 0;
+    for (int k = 0; k < kSmall; ++k) {
+        T* rb = rows[k & 1];
+        if (i == k) {
 #pragma unroll
-    for (int i = 0; i < kSmall; ++i) {
-        if (i < nb) {
-            T acc = x[i];
+            for (int q = 0; q < 16; ++q) rb[w + 4 * q] = a[q];
+        }
+        __syncthreads();
+        const T piv = rb[k];
+        if (!(piv > T(0))) {  // the same value in every thread: a uniform exit
+            fail = k + 1;
+            break;
+        }
+        if (t == 0) diag[k] = piv;
+        // Branch-free update (selects, not branches, keep a[] in place):
+        // rows i <= k keep their values.
+        const T mi = rb[i] / piv;
 #pragma unroll
-            for (int k = 0; k < i; ++k) acc -= s[k][i] * x[k];
-            x[i] = unit ? acc : acc / s[i][i];
+        for (int q = 0; q < 16; ++q) {
+            const int j = w + 4 * q;
+            const T r = rb[j];  // read unconditionally: no branch around the load
+            const T f = (j >= i || j < k) ? r : (j == k ? T(1) : T(0));
+            const T upd = a[q] - mi * f;
+            a[q] = (i > k) ? upd : a[q];
         }
     }
+    if (fail) {
+        if (t == 0) atomicCAS(info, 0, base + fail);
+        return;
+    }
+    __syncthreads();
+    const T si = sqrt(diag[i]);
 #pragma unroll
-    for (int i = 0; i < kSmall; ++i)
-        if (i < nb) b[i] = x[i];
+    for (int q = 0; q < 16; ++q) {
+        const int j = w + 4 * q;
+        if (i <= j && j < nb) A[i + (size_t)j * lda] = a[q] / si;
+        // Linv(i, j) = Ut^{-T}(i, j) / sqrt(d_i), unit diagonal of Ut^{-T}.
+        Linv[i + kSmall * j] = (j < i) ? a[q] / si : (j == i ? T(1) / si : T(0));
+    }
 }
 
-// B := U^{-1} B  (backward substitution), B nb x ncols.
+// Linv blocks of an existing upper triangle of order m: workgroup b inverts
+// diagonal block b (order nb = min(64, m - 64 b), unit diagonal if `unit`)
+// by back substitution, thread j solving U_bb x = e_j, and stores the
+// transpose through LDS so the global stores are coalesced.
 template <typename T>
-__global__ __launch_bounds__(256) void trsm_lun_small_kernel(int nb, int ncols,
-                                                              const T* __restrict__ A, int lda,
-                                                              T* __restrict__ B, int ldb,
-                                                              int unit,
-                                                              const int* __restrict__ stop) {
+__global__ __launch_bounds__(64) void inv_upper_blocks_kernel(int m, const T* __restrict__ A,
+                                                               int lda, T* __restrict__ Linv,
+                                                               int unit,
+                                                               const int* __restrict__ stop) {
     __shared__ T s[kSmall][kSmall + 1];
     if (stop && *stop) return;
-    load_upper_lds<T>(s, nb, A, lda);
+    const int b = blockIdx.x, nb = min(kSmall, m - kSmall * b);
+    const T* Ab = A + (size_t)kSmall * b * (1 + (size_t)lda);
+    for (int e = threadIdx.x; e < kSmall * kSmall; e += 64) {
+        const int i = e % kSmall, c = e / kSmall;
+        s[i][c] = (i <= c && c < nb) ? Ab[i + (size_t)c * lda] : T(0);
+    }
     __syncthreads();
-    int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= ncols) return;
-    T* b = B + (size_t)col * ldb;
+    const int j = threadIdx.x;
     T x[kSmall];
-#pragma unroll
-    for (int i = 0; i < kSmall; ++i) x[i] = (i < nb) ? b[i] : T(0);
 #pragma unroll
     for (int i = kSmall - 1; i >= 0; --i) {
-        if (i < nb) {
-            T acc = x[i];
+        T acc = (i == j) ? T(1) : T(0);
 #pragma unroll
-            for (int k = i + 1; k < kSmall; ++k)
-                if (k < nb) acc -= s[i][k] * x[k];
-            x[i] = unit ? acc : acc / s[i][i];
-        }
+        for (int k = i + 1; k < kSmall; ++k) acc -= s[i][k] * x[k];
+        // Rows past nb are the identity padding.
+        x[i] = (i < nb && j < nb) ? (unit ? acc : acc / s[i][i]) : acc;
     }
+    __syncthreads();
 #pragma unroll
-    for (int i = 0; i < kSmall; ++i)
-        if (i < nb) b[i] = x[i];
+    for (int i = 0; i < kSmall; ++i) s[i][j] = x[i];  // s = U_bb^{-1}
+    __syncthreads();
+    T* Lb = Linv + (size_t)4096 * b;
+    for (int e = threadIdx.x; e < kSmall * kSmall; e += 64) {
+        const int r = e % kSmall, c = e / kSmall;
+        Lb[e] = s[c][r];
+    }
 }
 
-// B := B U^{-1}, B nrows x nb (one row per thread; coalesced across threads).
-template <typename T>
-__global__ __launch_bounds__(256) void trsm_run_small_kernel(int nb, int nrows,
-                                                              const T* __restrict__ A, int lda,
-                                                              T* __restrict__ B, int ldb,
-                                                              int unit,
-                                                              const int* __restrict__ stop) {
-    __shared__ T s[kSmall][kSmall + 1];
+// Leaf of the blocked triangular solve, C = op(A) op(B) on one slab of at
+// most 64 rows (side L) or 64 columns (side R), in place: C is the same
+// memory as B (side L) or A (side R).  That is safe because the slab's
+// 64-wide dimension is one tile: each workgroup reads its whole input tile
+// (tile_mma ends in a barrier after its last load) before writing the same
+// tile, and no other workgroup touches it.  The pointers are deliberately
+// not __restrict__.
+template <typename T, bool TA, bool TB>
+__global__ __launch_bounds__(256) void trsm_leaf_kernel(int m, int n, int k, const T* A, int lda,
+                                                         const T* B, int ldb, T* C, int ldc,
+                                                         const int* stop) {
+    __shared__ T sA[kTileBK][kSmall + kTilePad];
+    __shared__ T sB[kTileBK][kSmall + kTilePad];
     if (stop && *stop) return;
-    load_upper_lds<T>(s, nb, A, lda);
-    __syncthreads();
-    int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= nrows) return;
-    T x[kSmall];
+    const int m0 = blockIdx.x * kSmall, n0 = blockIdx.y * kSmall;
+    typename Mfma<T>::acc_t acc[2][2];
+    zero_acc<T, 2, 2>(acc);
+    tile_mma<T, kSmall, kSmall, TA, TB>(acc, sA, sB, m, n, m0, n0, 0, k, A, lda, B, ldb);
 #pragma unroll
-    for (int j = 0; j < kSmall; ++j) x[j] = (j < nb) ? B[row + (size_t)j * ldb] : T(0);
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int j = 0; j < kSmall; ++j) {
-        if (j < nb) {
-            T acc = x[j];
+        for (int b = 0; b < 2; ++b)
 #pragma unroll
-            for (int k = 0; k < j; ++k) acc -= x[k] * s[k][j];
-            x[j] = unit ? acc : acc / s[j][j];
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < kSmall; ++j)
-        if (j < nb) B[row + (size_t)j * ldb] = x[j];
+            for (int r = 0; r < 4; ++r) {
+                const int i = m0 + tile_row<T, kSmall, kSmall>(a, r);
+                const int j = n0 + tile_col<kSmall, kSmall>(b);
+                if (i < m && j < n) C[i + (size_t)j * ldc] = acc[a][b][r];
+            }
 }
 
-// Unblocked right-looking upper Cholesky of an nb <= 64 diagonal block in LDS.
-// On a non-positive (or NaN) pivot at local step kk, stores base + kk + 1 in
-// *info (first failure wins) and leaves the block partially factored.
+// B slab := (Linv block) applied as the solve with diagonal block `blk`;
+// `order` <= 64 is the block's order, `count` the other dimension.
 template <typename T>
-__global__ __launch_bounds__(256) void potrf_small_kernel(int nb, T* __restrict__ A, int lda,
-                                                           int* __restrict__ info, int base) {
-    __shared__ T s[kSmall][kSmall + 1];
-    __shared__ int fail;
-    if (*info != 0) return;
-    load_upper_lds<T>(s, nb, A, lda);
-    if (threadIdx.x == 0) fail = 0;
-    __syncthreads();
-    for (int kk = 0; kk < nb; ++kk) {
-        if (threadIdx.x == 0) {
-            T dkk = s[kk][kk];
-            if (!(dkk > T(0)))
-                fail = kk + 1;
-            else
-                s[kk][kk] = sqrt(dkk);
-        }
-        __syncthreads();
-        if (fail) break;
-        T piv = s[kk][kk];
-        for (int j = kk + 1 + threadIdx.x; j < nb; j += blockDim.x) s[kk][j] /= piv;
-        __syncthreads();
-        int w = nb - kk - 1;
-        for (int e = threadIdx.x; e < w * w; e += blockDim.x) {
-            int i = kk + 1 + e % w, j = kk + 1 + e / w;
-            if (i <= j) s[i][j] -= s[kk][i] * s[kk][j];
-        }
-        __syncthreads();
+static int trsm_leaf(dh_handle_t h, char side, char trans, int order, int count, const T* Lb,
+                     T* B, int ldb, const int* stop) {
+    if (side == 'L') {
+        dim3 grid(1, dh_cdiv(count, kSmall));
+        if (trans == 'T')
+            trsm_leaf_kernel<T, false, false><<<grid, 256, 0, h->stream>>>(
+                order, count, order, Lb, kSmall, B, ldb, B, ldb, stop);
+        else
+            trsm_leaf_kernel<T, true, false><<<grid, 256, 0, h->stream>>>(
+                order, count, order, Lb, kSmall, B, ldb, B, ldb, stop);
+    } else {
+        dim3 grid(dh_cdiv(count, kSmall), 1);
+        trsm_leaf_kernel<T, false, true><<<grid, 256, 0, h->stream>>>(
+            count, order, order, B, ldb, Lb, kSmall, B, ldb, stop);
     }
-    for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
-        int i = e % nb, j = e / nb;
-        if (i <= j) A[i + (size_t)j * lda] = s[i][j];
-    }
-    if (threadIdx.x == 0 && fail) atomicCAS(info, 0, base + fail);
+    DH_CHECK_LAUNCH();
+    return 0;
 }
 
 __global__ void set_int_kernel(int* p, int v) { *p = v; }
 
 // --------------------------------------------------------- TRSM drivers
+//
+// Recursive halving as triangular_solve (blas_like.h:113-122), split at a
+// multiple of 64 so every leaf is one 64-row (64-column) diagonal block,
+// and each leaf is an MFMA product with that block's Linv instead of a
+// substitution.  `D` holds the Linv blocks of the whole triangle passed at
+// the top; `blk` is the index of the current sub-triangle's first block.
 
 template <typename T>
-static int trsm_rec(dh_handle_t h, char side, char trans, bool unit, int m, int n, const T* A,
-                    int lda, T* B, int ldb, const int* stop) {
+static int trsm_rec(dh_handle_t h, char side, char trans, int m, int n, const T* A, int lda,
+                    T* B, int ldb, const T* D, int blk, const int* stop) {
     if (m == 0 || n == 0) return 0;
+    const int order = side == 'L' ? m : n;
+    if (order <= kSmall)
+        return trsm_leaf<T>(h, side, trans, order, side == 'L' ? n : m, D + (size_t)4096 * blk, B,
+                            ldb, stop);
+    const int h1 = ((order / 2 + kSmall - 1) / kSmall) * kSmall, h2 = order - h1;
+    const int b2 = blk + h1 / kSmall;
     if (side == 'L' && trans == 'T') {
-        // U^T X = B: halve like triangular_solve (blas_like.h:113-122).
-        if (m <= kSmall) {
-            trsm_lut_small_kernel<T><<<dh_cdiv(n, 256), 256, 0, h->stream>>>(m, n, A, lda, B, ldb,
-                                                                              unit, stop);
-            DH_CHECK_LAUNCH();
-            return 0;
-        }
-        int m1 = m / 2, m2 = m - m1;
-        DH_CHECK_STATUS(trsm_rec<T>(h, side, trans, unit, m1, n, A, lda, B, ldb, stop));
-        DH_CHECK_STATUS(gemm_stop<T>(h, true, false, m2, n, m1, T(-1), A + (size_t)m1 * lda, lda,
-                                     B, ldb, T(1), B + m1, ldb, 0, stop));
-        return trsm_rec<T>(h, side, trans, unit, m2, n, A + m1 + (size_t)m1 * lda, lda, B + m1,
-                           ldb, stop);
+        // U^T X = B: top half, update, bottom half.
+        DH_CHECK_STATUS(trsm_rec<T>(h, side, trans, h1, n, A, lda, B, ldb, D, blk, stop));
+        DH_CHECK_STATUS(gemm_stop<T>(h, true, false, h2, n, h1, T(-1), A + (size_t)h1 * lda, lda,
+                                     B, ldb, T(1), B + h1, ldb, 0, stop));
+        return trsm_rec<T>(h, side, trans, h2, n, A + h1 + (size_t)h1 * lda, lda, B + h1, ldb, D,
+                           b2, stop);
     }
     if (side == 'L') {
         // U X = B: bottom half first.
-        if (m <= kSmall) {
-            trsm_lun_small_kernel<T><<<dh_cdiv(n, 256), 256, 0, h->stream>>>(m, n, A, lda, B, ldb,
-                                                                              unit, stop);
-            DH_CHECK_LAUNCH();
-            return 0;
-        }
-        int m1 = m / 2, m2 = m - m1;
-        DH_CHECK_STATUS(trsm_rec<T>(h, side, trans, unit, m2, n, A + m1 + (size_t)m1 * lda, lda,
-                                    B + m1, ldb, stop));
-        DH_CHECK_STATUS(gemm_stop<T>(h, false, false, m1, n, m2, T(-1), A + (size_t)m1 * lda,
-                                     lda, B + m1, ldb, T(1), B, ldb, 0, stop));
-        return trsm_rec<T>(h, side, trans, unit, m1, n, A, lda, B, ldb, stop);
+        DH_CHECK_STATUS(trsm_rec<T>(h, side, trans, h2, n, A + h1 + (size_t)h1 * lda, lda, B + h1,
+                                    ldb, D, b2, stop));
+        DH_CHECK_STATUS(gemm_stop<T>(h, false, false, h1, n, h2, T(-1), A + (size_t)h1 * lda,
+                                     lda, B + h1, ldb, T(1), B, ldb, 0, stop));
+        return trsm_rec<T>(h, side, trans, h1, n, A, lda, B, ldb, D, blk, stop);
     }
     // side R, no transpose: X U = B, triangle of order n, B m x n.
-    if (n <= kSmall) {
-        trsm_run_small_kernel<T><<<dh_cdiv(m, 256), 256, 0, h->stream>>>(n, m, A, lda, B, ldb,
-                                                                          unit, stop);
-        DH_CHECK_LAUNCH();
-        return 0;
-    }
-    int n1 = n / 2, n2 = n - n1;
-    DH_CHECK_STATUS(trsm_rec<T>(h, side, trans, unit, m, n1, A, lda, B, ldb, stop));
-    DH_CHECK_STATUS(gemm_stop<T>(h, false, false, m, n2, n1, T(-1), B, ldb, A + (size_t)n1 * lda,
-                                 lda, T(1), B + (size_t)n1 * ldb, ldb, 0, stop));
-    return trsm_rec<T>(h, side, trans, unit, m, n2, A + n1 + (size_t)n1 * lda, lda,
-                       B + (size_t)n1 * ldb, ldb, stop);
+    DH_CHECK_STATUS(trsm_rec<T>(h, side, trans, m, h1, A, lda, B, ldb, D, blk, stop));
+    DH_CHECK_STATUS(gemm_stop<T>(h, false, false, m, h2, h1, T(-1), B, ldb, A + (size_t)h1 * lda,
+                                 lda, T(1), B + (size_t)h1 * ldb, ldb, 0, stop));
+    return trsm_rec<T>(h, side, trans, m, h2, A + h1 + (size_t)h1 * lda, lda,
+                       B + (size_t)h1 * ldb, ldb, D, b2, stop);
+}
+
+// Triangular solve with the upper triangle A (order m for side L, n for
+// side R): Linv of its diagonal blocks, then the recursion.
+template <typename T>
+static int trsm_blocked(dh_handle_t h, char side, char trans, bool unit, int m, int n, const T* A,
+                        int lda, T* B, int ldb, const int* stop) {
+    if (m == 0 || n == 0) return 0;
+    const int order = side == 'L' ? m : n;
+    const int nblk = dh_cdiv(order, kSmall);
+    T* D = (T*)dh_ws(h, DH_WS_TRSM_INV, sizeof(T) * 4096 * (size_t)nblk);
+    if (!D) return DH_ERR_ALLOC;
+    inv_upper_blocks_kernel<T><<<nblk, 64, 0, h->stream>>>(order, A, lda, D, unit ? 1 : 0, stop);
+    DH_CHECK_LAUNCH();
+    return trsm_rec<T>(h, side, trans, m, n, A, lda, B, ldb, D, 0, stop);
 }
 
 template <typename T>
@@ -382,7 +409,7 @@ int dh_trsm_impl(dh_handle_t h, char side, char trans, bool unit, int m, int n, 
         DH_CHECK_LAUNCH();
         if (alpha == T(0)) return 0;
     }
-    return trsm_rec<T>(h, side, trans, unit, m, n, A, lda, B, ldb, nullptr);
+    return trsm_blocked<T>(h, side, trans, unit, m, n, A, lda, B, ldb, nullptr);
 }
 template int dh_trsm_impl<double>(dh_handle_t, char, char, bool, int, int, double, const double*,
                                   int, double*, int);
@@ -392,19 +419,20 @@ template int dh_trsm_impl<float>(dh_handle_t, char, char, bool, int, int, float,
 // --------------------------------------------------------- Cholesky driver
 
 // Factor one diagonal block of order b (<= any): right-looking in 64-wide
-// steps entirely on the device; failures land in *dinfo with global index.
+// steps entirely on the device, leaving the Linv of each 64-block in D;
+// failures land in *dinfo with their global index.
 template <typename T>
-static int potrf_block(dh_handle_t h, int b, T* A11, int lda, int* dinfo, int base) {
+static int potrf_block(dh_handle_t h, int b, T* A11, int lda, T* D, int* dinfo, int base) {
     for (int ii = 0; ii < b; ii += kSmall) {
         int bb = min(kSmall, b - ii);
         T* Aii = A11 + ii + (size_t)ii * lda;
-        potrf_small_kernel<T><<<1, 256, 0, h->stream>>>(bb, Aii, lda, dinfo, base + ii);
+        T* Lb = D + (size_t)4096 * (ii / kSmall);
+        potrf_inv_small_kernel<T><<<1, 256, 0, h->stream>>>(bb, Aii, lda, Lb, dinfo, base + ii);
         DH_CHECK_LAUNCH();
         if (ii + bb < b) {
             T* A12 = A11 + ii + (size_t)(ii + bb) * lda;
             T* A22 = A11 + (ii + bb) + (size_t)(ii + bb) * lda;
-            DH_CHECK_STATUS(trsm_rec<T>(h, 'L', 'T', false, bb, b - ii - bb, Aii, lda, A12, lda,
-                                        dinfo));
+            DH_CHECK_STATUS(trsm_leaf<T>(h, 'L', 'T', bb, b - ii - bb, Lb, A12, lda, dinfo));
             DH_CHECK_STATUS(gemm_stop<T>(h, true, false, b - ii - bb, b - ii - bb, bb, T(-1), A12,
                                          lda, A12, lda, T(1), A22, lda, 1, dinfo));
         }
@@ -419,6 +447,9 @@ int dh_potrf_impl(dh_handle_t h, int n, T* A, int lda, int bs, int* info_host) {
     if (bs <= 0) bs = 256;
     int* dinfo = (int*)dh_ws(h, DH_WS_INFO, 64);
     if (!dinfo) return DH_ERR_ALLOC;
+    // Linv blocks of the current panel's diagonal block.
+    T* D = (T*)dh_ws(h, DH_WS_TRSM_INV, sizeof(T) * 4096 * (size_t)dh_cdiv(min(bs, n), kSmall));
+    if (!D) return DH_ERR_ALLOC;
     set_int_kernel<<<1, 1, 0, h->stream>>>(dinfo, 0);
     DH_CHECK_LAUNCH();
     // cholesky_factorize (lapack_like.h:17-52): potrf(A11); trsm("L","U","T") on
@@ -426,11 +457,12 @@ int dh_potrf_impl(dh_handle_t h, int n, T* A, int lda, int bs, int* info_host) {
     for (int i = 0; i < n; i += bs) {
         int b = min(bs, n - i);
         T* A11 = A + i + (size_t)i * lda;
-        DH_CHECK_STATUS(potrf_block<T>(h, b, A11, lda, dinfo, i));
+        DH_CHECK_STATUS(potrf_block<T>(h, b, A11, lda, D, dinfo, i));
         if (i + b < n) {
             T* A12 = A + i + (size_t)(i + b) * lda;
             T* A22 = A + (i + b) + (size_t)(i + b) * lda;
-            DH_CHECK_STATUS(trsm_rec<T>(h, 'L', 'T', false, b, n - i - b, A11, lda, A12, lda, dinfo));
+            DH_CHECK_STATUS(trsm_rec<T>(h, 'L', 'T', b, n - i - b, A11, lda, A12, lda, D, 0,
+                                        dinfo));
             DH_CHECK_STATUS(gemm_stop<T>(h, true, false, n - i - b, n - i - b, b, T(-1), A12, lda,
                                          A12, lda, T(1), A22, lda, 1, dinfo));
         }
