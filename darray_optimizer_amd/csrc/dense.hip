@@ -198,10 +198,13 @@ template <typename T>
 __global__ __launch_bounds__(256) void potrf_inv_small_kernel(int nb, T* __restrict__ A, int lda,
                                                                T* __restrict__ Linv,
                                                                int* __restrict__ info, int base) {
+    // Row k is broadcast permuted, column j at (j % 4) * 16 + j / 4, so the
+    // 16 columns a thread holds are contiguous (vector LDS reads/writes).
     __shared__ T rows[2][kSmall];
     __shared__ T diag[kSmall];
     if (*info != 0) return;
     const int t = threadIdx.x, i = t & (kSmall - 1), w = t >> 6;
+    const int pi = (i & 3) * 16 + (i >> 2);
     T a[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
@@ -214,26 +217,31 @@ __global__ __launch_bounds__(256) void potrf_inv_small_kernel(int nb, T* __restr
         T* rb = rows[k & 1];
         if (i == k) {
 #pragma unroll
-            for (int q = 0; q < 16; ++q) rb[w + 4 * q] = a[q];
+            for (int q = 0; q < 16; ++q) rb[w * 16 + q] = a[q];
         }
         __syncthreads();
-        const T piv = rb[k];
+        // All reads of the step at once; the pivot test comes after the
+        // update (a failed factorisation writes nothing back).
+        const T piv = rb[(k & 3) * 16 + (k >> 2)];
+        const T rbi = rb[pi];
+        T rj[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) rj[q] = rb[w * 16 + q];
+        // Branch-free update (selects keep a[] in place): rows i <= k keep
+        // their values.
+        const T mi = rbi / piv;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int j = w + 4 * q;
+            const T f = (j >= i || j < k) ? rj[q] : (j == k ? T(1) : T(0));
+            const T upd = a[q] - mi * f;
+            a[q] = (i > k) ? upd : a[q];
+        }
         if (!(piv > T(0))) {  // the same value in every thread: a uniform exit
             fail = k + 1;
             break;
         }
         if (t == 0) diag[k] = piv;
-        // Branch-free update (selects, not branches, keep a[] in place):
-        // rows i <= k keep their values.
-        const T mi = rb[i] / piv;
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int j = w + 4 * q;
-            const T r = rb[j];  // read unconditionally: no branch around the load
-            const T f = (j >= i || j < k) ? r : (j == k ? T(1) : T(0));
-            const T upd = a[q] - mi * f;
-            a[q] = (i > k) ? upd : a[q];
-        }
     }
     if (fail) {
         if (t == 0) atomicCAS(info, 0, base + fail);
@@ -263,7 +271,9 @@ __global__ __launch_bounds__(64) void inv_upper_blocks_kernel(int m, const T* __
     if (stop && *stop) return;
     const int b = blockIdx.x, nb = min(kSmall, m - kSmall * b);
     const T* Ab = A + (size_t)kSmall * b * (1 + (size_t)lda);
-    for (int e = threadIdx.x; e < kSmall * kSmall; e += 64) {
+#pragma unroll 16
+    for (int u = 0; u < kSmall; ++u) {
+        const int e = threadIdx.x + 64 * u;
         const int i = e % kSmall, c = e / kSmall;
         s[i][c] = (i <= c && c < nb) ? Ab[i + (size_t)c * lda] : T(0);
     }
@@ -283,7 +293,9 @@ __global__ __launch_bounds__(64) void inv_upper_blocks_kernel(int m, const T* __
     for (int i = 0; i < kSmall; ++i) s[i][j] = x[i];  // s = U_bb^{-1}
     __syncthreads();
     T* Lb = Linv + (size_t)4096 * b;
-    for (int e = threadIdx.x; e < kSmall * kSmall; e += 64) {
+#pragma unroll 16
+    for (int u = 0; u < kSmall; ++u) {
+        const int e = threadIdx.x + 64 * u;
         const int r = e % kSmall, c = e / kSmall;
         Lb[e] = s[c][r];
     }
@@ -337,6 +349,184 @@ static int trsm_leaf(dh_handle_t h, char side, char trans, int order, int count,
         trsm_leaf_kernel<T, false, true><<<grid, 256, 0, h->stream>>>(
             count, order, order, B, ldb, Lb, kSmall, B, ldb, stop);
     }
+    DH_CHECK_LAUNCH();
+    return 0;
+}
+
+// --------------------------------------------------- block-sweep TRSM
+//
+// For few right-hand sides the recursion's leaves and updates are too
+// small to fill the GPU, so side-L solves with up to kSweepMaxCols columns
+// run as a sweep over the 64-row blocks, one launch per block:
+//   forward  (U^T X = B): X_i = Linv_i B_i,    B_j -= U_ij^T X_i  (j > i)
+//   backward (U X = B):   X_i = Linv_i^T B_i,  B_j -= U_ji X_i    (j < i)
+// Workgroup (b, c) owns block j = i +- b and the 64 columns of tile c; every
+// workgroup recomputes X_i for its tile (64^3 multiply-adds, cheaper than a
+// second launch), workgroup (0, c) stores it to X, and the others update
+// their B_j.  B_i itself is read again by no later launch, so nothing
+// waits on a store made in the same launch.  X is copied back to B at the
+// end.
+
+constexpr int kSweepMaxCols = 512;
+constexpr int kLdsPad = kSmall + 1;
+
+// acc += op(sA) sB over k = 0..63, both operands in LDS:
+// op(sA)[r][k] = TA ? sA[k][r] : sA[r][k].  Waves in a 2 x 2 grid, each
+// owning a 32 x 32 quarter as 2 x 2 MFMA blocks (as tile_mma).
+template <typename T, bool TA>
+__device__ __forceinline__ void lds_mma64(typename Mfma<T>::acc_t (&acc)[2][2],
+                                          const T (*sA)[kLdsPad], const T (*sB)[kLdsPad]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1, g = lane >> 4, li = lane & 15;
+#pragma unroll 4
+    for (int ks = 0; ks < kSmall / 4; ++ks) {
+        const int k = 4 * ks + g;
+        T av[2], bv[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const int r = wm * 32 + a * 16 + li;
+            av[a] = TA ? sA[k][r] : sA[r][k];
+        }
+#pragma unroll
+        for (int b = 0; b < 2; ++b) bv[b] = sB[k][wn * 32 + b * 16 + li];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) acc[a][b] = Mfma<T>::op(av[a], bv[b], acc[a][b]);
+    }
+}
+
+template <typename T, bool BACK>
+__global__ __launch_bounds__(256) void trsm_sweep_kernel(int m, int n, int i,
+                                                          const T* __restrict__ U, int ldu,
+                                                          const T* __restrict__ Linv,
+                                                          T* __restrict__ B, int ldb,
+                                                          T* __restrict__ X, int ldx,
+                                                          const int* __restrict__ stop) {
+    __shared__ T sA[kSmall][kLdsPad];  // Linv_i
+    __shared__ T sB[kSmall][kLdsPad];  // B_i, then X_i
+    __shared__ T sU[kSmall][kLdsPad];  // coupling block
+    if (stop && *stop) return;
+    const int b = blockIdx.x, c0 = blockIdx.y * kSmall;
+    const int j = BACK ? i - b : i + b;
+    const int r0 = kSmall * i, rj = kSmall * j;
+    const T* Lb = Linv + (size_t)4096 * i;
+    // Every operand is loaded up front (one memory round trip per launch):
+    // Linv_i (identity-padded), B_i (zero past row m / column n), the
+    // coupling block -- forward U_ij (rows of block i, columns of block j,
+    // used transposed), backward U_ji -- and this workgroup's B_j tile in
+    // the accumulator layout.  All loads are issued into registers first,
+    // from addresses clamped into the matrices, and stored to LDS after.
+    T la[16], lb[16], lu[16], bj[2][2][4];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int e = threadIdx.x + 256 * u;
+        const int r = e % kSmall, c = e / kSmall;
+        la[u] = Lb[e];
+        lb[u] = B[min(r0 + r, m - 1) + (size_t)min(c0 + c, n - 1) * ldb];
+    }
+    if (b != 0) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int e = threadIdx.x + 256 * u;
+            const int r = e % kSmall, c = e / kSmall;
+            const int ur = min(BACK ? rj + r : r0 + r, m - 1);
+            const int uc = min(BACK ? r0 + c : rj + c, m - 1);
+            lu[u] = U[ur + (size_t)uc * ldu];
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = min(rj + tile_row<T, kSmall, kSmall>(a, r), m - 1);
+                    const int col = min(c0 + tile_col<kSmall, kSmall>(bb), n - 1);
+                    bj[a][bb][r] = B[row + (size_t)col * ldb];
+                }
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int e = threadIdx.x + 256 * u;
+        const int r = e % kSmall, c = e / kSmall;
+        sA[r][c] = la[u];
+        sB[r][c] = (r0 + r < m && c0 + c < n) ? lb[u] : T(0);
+        if (b != 0) {
+            const bool ok = BACK ? (rj + r < m && r0 + c < m) : (r0 + r < m && rj + c < m);
+            sU[r][c] = ok ? lu[u] : T(0);
+        }
+    }
+    __syncthreads();
+    typename Mfma<T>::acc_t acc[2][2];
+    zero_acc<T, 2, 2>(acc);
+    lds_mma64<T, BACK>(acc, sA, sB);
+    __syncthreads();
+    // X_i tile into sB (block row major, as the next product reads it).
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                sB[tile_row<T, kSmall, kSmall>(a, r)][tile_col<kSmall, kSmall>(bb)] =
+                    acc[a][bb][r];
+    __syncthreads();
+    if (b == 0) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int e = threadIdx.x + 256 * u;
+            const int r = e % kSmall, c = e / kSmall;
+            if (r0 + r < m && c0 + c < n) X[(r0 + r) + (size_t)(c0 + c) * ldx] = sB[r][c];
+        }
+        return;
+    }
+    zero_acc<T, 2, 2>(acc);
+    lds_mma64<T, !BACK>(acc, sU, sB);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = rj + tile_row<T, kSmall, kSmall>(a, r);
+                const int col = c0 + tile_col<kSmall, kSmall>(bb);
+                if (row < m && col < n) B[row + (size_t)col * ldb] = bj[a][bb][r] - acc[a][bb][r];
+            }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void copy_cols_kernel(int m, int n, const T* __restrict__ S,
+                                                         int lds, T* __restrict__ D, int ldd,
+                                                         const int* __restrict__ stop) {
+    if (stop && *stop) return;
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (size_t)m * n) return;
+    const int r = (int)(idx % m), c = (int)(idx / m);
+    D[r + (size_t)c * ldd] = S[r + (size_t)c * lds];
+}
+
+// Side-L solve of U^T X = B (trans 'T') or U X = B (trans 'N'), U of order
+// m with Linv blocks in D, B m x n with n <= kSweepMaxCols.
+template <typename T>
+static int trsm_sweep(dh_handle_t h, char trans, int m, int n, const T* A, int lda, const T* D,
+                      T* B, int ldb, const int* stop) {
+    const int nblk = dh_cdiv(m, kSmall), tiles = dh_cdiv(n, kSmall);
+    T* X = (T*)dh_ws(h, DH_WS_TRSM_X, sizeof(T) * (size_t)m * n);
+    if (!X) return DH_ERR_ALLOC;
+    for (int s = 0; s < nblk; ++s) {
+        if (trans == 'T') {
+            const int i = s;
+            trsm_sweep_kernel<T, false><<<dim3(nblk - i, tiles), 256, 0, h->stream>>>(
+                m, n, i, A, lda, D, B, ldb, X, m, stop);
+        } else {
+            const int i = nblk - 1 - s;
+            trsm_sweep_kernel<T, true><<<dim3(i + 1, tiles), 256, 0, h->stream>>>(
+                m, n, i, A, lda, D, B, ldb, X, m, stop);
+        }
+        DH_CHECK_LAUNCH();
+    }
+    copy_cols_kernel<T><<<dh_cdiv((long)m * n, 256), 256, 0, h->stream>>>(m, n, X, m, B, ldb,
+                                                                          stop);
     DH_CHECK_LAUNCH();
     return 0;
 }
@@ -397,6 +587,8 @@ static int trsm_blocked(dh_handle_t h, char side, char trans, bool unit, int m, 
     if (!D) return DH_ERR_ALLOC;
     inv_upper_blocks_kernel<T><<<nblk, 64, 0, h->stream>>>(order, A, lda, D, unit ? 1 : 0, stop);
     DH_CHECK_LAUNCH();
+    if (side == 'L' && m > kSmall && n <= kSweepMaxCols)
+        return trsm_sweep<T>(h, trans, m, n, A, lda, D, B, ldb, stop);
     return trsm_rec<T>(h, side, trans, m, n, A, lda, B, ldb, D, 0, stop);
 }
 

@@ -25,6 +25,7 @@ enum dh_slot {
     DH_WS_SOLVE_A,     // low-precision copy of a matrix being factored
     DH_WS_SOLVE_R,     // residual / correction panels of iterative refinement
     DH_WS_TRSM_INV,    // inverses of the 64 x 64 diagonal blocks of a triangular solve
+    DH_WS_TRSM_X,      // solution blocks of the block-sweep triangular solve
     DH_WS_COUNT
 };
 
