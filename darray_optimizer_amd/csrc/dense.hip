@@ -645,20 +645,57 @@ int dh_potrf_impl(dh_handle_t h, int n, T* A, int lda, int bs, int* info_host) {
     set_int_kernel<<<1, 1, 0, h->stream>>>(dinfo, 0);
     DH_CHECK_LAUNCH();
     // cholesky_factorize (lapack_like.h:17-52): potrf(A11); trsm("L","U","T") on
-    // A12; upper SYRK A22 -= A12^T A12.
+    // A12; upper SYRK A22 -= A12^T A12.  The SYRK is split with a look-ahead
+    // of one panel: the rows of the next panel are updated on the handle's
+    // stream, which then factors that panel (latency-bound one-workgroup
+    // kernels and its panel TRSM) while the rest of the trailing matrix is
+    // updated on a second stream.
+    //   stream:  ... potrf/trsm(p), record P | wait U(p-1), update rows of p+1 | potrf/trsm(p+1) ...
+    //   aux:     wait P, update the rest after p, record U(p) | ...
+    // Regions: the aux update writes rows past panel p+1; panel p+1's
+    // factorisation reads and writes rows of p+1 only, and U12 of panel p is
+    // no longer written by anyone.
+    const bool lookahead = n > bs;
+    if (lookahead) {
+        DH_CHECK_STATUS(dh_aux_stream(h));
+        DH_CHECK_HIP(hipEventRecord(h->ev_panel, h->stream));  // aux after prior work
+        DH_CHECK_HIP(hipStreamWaitEvent(h->aux, h->ev_panel, 0));
+    }
+    bool pending = false;  // an aux update of an earlier panel was queued
     for (int i = 0; i < n; i += bs) {
         int b = min(bs, n - i);
         T* A11 = A + i + (size_t)i * lda;
         DH_CHECK_STATUS(potrf_block<T>(h, b, A11, lda, D, dinfo, i));
-        if (i + b < n) {
-            T* A12 = A + i + (size_t)(i + b) * lda;
-            T* A22 = A + (i + b) + (size_t)(i + b) * lda;
-            DH_CHECK_STATUS(trsm_rec<T>(h, 'L', 'T', b, n - i - b, A11, lda, A12, lda, D, 0,
-                                        dinfo));
-            DH_CHECK_STATUS(gemm_stop<T>(h, true, false, n - i - b, n - i - b, b, T(-1), A12, lda,
-                                         A12, lda, T(1), A22, lda, 1, dinfo));
+        if (i + b >= n) break;
+        T* A12 = A + i + (size_t)(i + b) * lda;
+        T* A22 = A + (i + b) + (size_t)(i + b) * lda;
+        const int nt = n - i - b;
+        DH_CHECK_STATUS(trsm_rec<T>(h, 'L', 'T', b, nt, A11, lda, A12, lda, D, 0, dinfo));
+        if (!lookahead) {
+            DH_CHECK_STATUS(gemm_stop<T>(h, true, false, nt, nt, b, T(-1), A12, lda, A12, lda,
+                                         T(1), A22, lda, 1, dinfo));
+            continue;
+        }
+        const int b2 = min(bs, nt), nr = nt - b2;
+        DH_CHECK_HIP(hipEventRecord(h->ev_panel, h->stream));
+        if (pending) DH_CHECK_HIP(hipStreamWaitEvent(h->stream, h->ev_update, 0));
+        // Rows of the next panel, all trailing columns (upper part only).
+        DH_CHECK_STATUS(gemm_stop<T>(h, true, false, b2, nt, b, T(-1), A12, lda, A12, lda, T(1),
+                                     A22, lda, 1, dinfo));
+        if (nr > 0) {
+            DH_CHECK_HIP(hipStreamWaitEvent(h->aux, h->ev_panel, 0));
+            hipStream_t main = h->stream;
+            h->stream = h->aux;  // gemm_stop launches on h->stream
+            const int st = gemm_stop<T>(h, true, false, nr, nr, b, T(-1), A12 + (size_t)b2 * lda,
+                                        lda, A12 + (size_t)b2 * lda, lda, T(1),
+                                        A22 + b2 + (size_t)b2 * lda, lda, 1, dinfo);
+            h->stream = main;
+            DH_CHECK_STATUS(st);
+            DH_CHECK_HIP(hipEventRecord(h->ev_update, h->aux));
+            pending = true;
         }
     }
+    if (pending) DH_CHECK_HIP(hipStreamWaitEvent(h->stream, h->ev_update, 0));
     int hinfo = 0;
     DH_CHECK_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     DH_CHECK_HIP(hipStreamSynchronize(h->stream));

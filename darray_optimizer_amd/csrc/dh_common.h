@@ -39,7 +39,14 @@ struct dh_handle_s {
     void* batch = nullptr;  // cached batched range-finder plan (batch.hip)
     void* tune = nullptr;   // autotuner decisions (batch.hip)
     void* timing = nullptr; // kernel timing events when enabled (handle.hip)
+    // Second stream (default priority) and two events for the look-ahead of
+    // the blocked Cholesky; created on first use, released by dh_destroy.
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_panel = nullptr, ev_update = nullptr;
 };
+
+// Creates h->aux and its events if they do not exist yet.
+int dh_aux_stream(dh_handle_t h);
 
 // Bracket one fused projection kernel launch with timing events (no-ops
 // unless dh_timing_enable switched timing on for the handle).

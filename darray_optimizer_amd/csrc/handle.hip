@@ -89,10 +89,23 @@ extern "C" int dh_destroy(dh_handle_t h) {
     dh_batch_release(h);
     dh_tune_release(h);
     dh_timing_enable(h, 0);
+    if (h->aux) (void)hipStreamSynchronize(h->aux);
     for (int s = 0; s < DH_WS_COUNT; ++s)
         if (h->ws[s]) (void)hipFree(h->ws[s]);
+    if (h->ev_panel) (void)hipEventDestroy(h->ev_panel);
+    if (h->ev_update) (void)hipEventDestroy(h->ev_update);
+    if (h->aux) (void)hipStreamDestroy(h->aux);
     if (h->own_stream) (void)hipStreamDestroy(h->stream);
     delete h;
+    return 0;
+}
+
+int dh_aux_stream(dh_handle_t h) {
+    if (h->aux) return 0;
+    // Default priority, like the handle's own stream.
+    DH_CHECK_HIP(hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking));
+    DH_CHECK_HIP(hipEventCreateWithFlags(&h->ev_panel, hipEventDisableTiming));
+    DH_CHECK_HIP(hipEventCreateWithFlags(&h->ev_update, hipEventDisableTiming));
     return 0;
 }
 
