@@ -167,7 +167,10 @@ int dh_trsm_f32(dh_handle_t h, char side, char uplo, char transa, char diag, int
  * (cholesky_factorize, lapack_like.h:9-56: potrf(A11) -> trsm(L,U,T) -> syrk).
  * The strict lower triangle is never written.  Returns 0 or info > 0, the
  * global 1-based order of the first non-positive pivot; A is then partially
- * factored (lapack_like.h:25-28).  Synchronises.  uplo must be 'U'. */
+ * factored (lapack_like.h:25-28).  Synchronises.  uplo must be 'U'.
+ * For n > bs part of each trailing update runs on a second stream owned by
+ * the handle (one-panel look-ahead); it is ordered after earlier work on the
+ * handle's stream and joined back before the call returns. */
 int dh_potrf_f64(dh_handle_t h, char uplo, int n, double* A, int lda, int bs);
 int dh_potrf_f32(dh_handle_t h, char uplo, int n, float* A, int lda, int bs);
 
