@@ -54,6 +54,7 @@ __device__ __forceinline__ void chol_small_body(int n, double* __restrict__ G, i
     __shared__ double s_tol, s_shift;
     __shared__ int s_info;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane >> 4, li = lane & 15;  // MFMA lane group and index
     constexpr int NW = NT / 64;
 
     {
@@ -159,49 +160,54 @@ __device__ __forceinline__ void chol_small_body(int n, double* __restrict__ G, i
             }
         }
         __syncthreads();
-        // A22 -= U12^T U12 on the upper triangle: 64 x 64 blocks, wave per block,
-        // lane = row (its NB panel values in registers), four columns per step.
-        const int wb = (w + 63) / 64;
+        // A22 -= U12^T U12 on the upper triangle in 16 x 16 blocks on FP64 MFMA
+        // (k = 16: four 16x16x4 steps, both operands from the LDS panel).
+        // Waves take blocks round robin; the four target values of the next
+        // block are loaded (clamped addresses, no branch) before this
+        // block's MFMAs.
+        const int wb = (w + 15) / 16;
         const int nblk = wb * (wb + 1) / 2;
-        for (int blk = wave; blk < nblk; blk += NW) {
-            int bi = 0, rem = blk;
+        double* A22 = G + (k0 + NB) + (size_t)(k0 + NB) * ldg;
+        auto block_of = [&](int blk, int& bi, int& bj) {
+            bi = 0;
+            int rem = blk;
             while (rem >= wb - bi) {
                 rem -= wb - bi;
                 ++bi;
             }
-            const int bj = bi + rem;
-            const int i = 64 * bi + lane;
-            double u[NB];
+            bj = bi + rem;
+        };
+        auto load_target = [&](int blk, double (&tv)[4]) {
+            int bi, bj;
+            block_of(min(blk, nblk - 1), bi, bj);
+            const int c = min(16 * bj + li, w - 1);
 #pragma unroll
-            for (int kk = 0; kk < NB; ++kk) u[kk] = (i < w) ? sP[kk][i] : 0.0;
-            const int jend = min(64 * bj + 64, w);
-            for (int j0 = 64 * bj; j0 < jend; j0 += 4) {
-                // Issue the loads of the four target elements before the FMAs.
-                double gv[4];
+            for (int r = 0; r < 4; ++r) {
+                const int row = min(16 * bi + g + 4 * r, w - 1);
+                tv[r] = A22[row + (size_t)c * ldg];
+            }
+        };
+        double tv[4];
+        load_target(wave, tv);
+        for (int blk = wave; blk < nblk; blk += NW) {
+            double tn[4];
+            load_target(blk + NW, tn);
+            int bi, bj;
+            block_of(blk, bi, bj);
+            dh_d4 acc = {0, 0, 0, 0};
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    int j = j0 + q;
-                    gv[q] = (i < w && j < w && i <= j)
-                                ? G[(k0 + NB + i) + (size_t)(k0 + NB + j) * ldg]
-                                : 0.0;
-                }
-                double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            for (int s4 = 0; s4 < NB / 4; ++s4) {
+                const int kk = 4 * s4 + g;
+                const double av = sP[kk][min(16 * bi + li, w - 1)];
+                const double bv = sP[kk][min(16 * bj + li, w - 1)];
+                acc = Mfma<double>::op(av, bv, acc);
+            }
+            const int c = 16 * bj + li;
 #pragma unroll
-                for (int kk = 0; kk < NB; ++kk) {
-                    const double2* row = reinterpret_cast<const double2*>(&sP[kk][j0]);
-                    double2 p01 = row[0], p23 = row[1];
-                    s0 += u[kk] * p01.x;
-                    s1 += u[kk] * p01.y;
-                    s2 += u[kk] * p23.x;
-                    s3 += u[kk] * p23.y;
-                }
-                double s[4] = {s0, s1, s2, s3};
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    int j = j0 + q;
-                    if (i < w && j < w && i <= j)
-                        G[(k0 + NB + i) + (size_t)(k0 + NB + j) * ldg] = gv[q] - s[q];
-                }
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * bi + g + 4 * r;  // accumulator r: row g + 4r, column li
+                if (row < w && c < w && row <= c) A22[row + (size_t)c * ldg] = tv[r] - acc[r];
+                tv[r] = tn[r];
             }
         }
         __syncthreads();
@@ -274,46 +280,60 @@ __device__ __forceinline__ void trsm_rows_body(int blk, int m, int n, double* __
     const int r0 = (blk * WAVES + wave) * 16;
     if (r0 >= m) return;
     const int nblk = (n + 15) / 16;
-    for (int J = 0; J < nblk; ++J) {
-        const int c0 = 16 * J, col = c0 + li;
+    // Y values and the U_JJ^{-1} column of block J are loaded one block ahead.
+    double yv[4], dv[4];
+    auto load_block = [&](int J, double (&y4)[4], double (&d4)[4]) {
+        const int col = 16 * J + li;
         const bool cok = col < n;
-        // Prefetch this block's Y values and U_JJ^{-1} column before the k loop.
-        double yv[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             int row = r0 + g + 4 * r;
-            yv[r] = (row < m && cok) ? Y[row + (size_t)col * ldy] : 0.0;
+            y4[r] = (row < m && cok) ? Y[row + (size_t)col * ldy] : 0.0;
         }
         const double* Di = Dinv + 256 * J;
-        double dv[4];
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) dv[ks] = Di[(4 * ks + g) + 16 * li];  // U_JJ^{-1}[4ks+g][li]
+        for (int ks = 0; ks < 4; ++ks) d4[ks] = Di[(4 * ks + g) + 16 * li];  // U_JJ^{-1}[4ks+g][li]
+    };
+    load_block(0, yv, dv);
+    for (int J = 0; J < nblk; ++J) {
+        const int c0 = 16 * J, col = c0 + li;
+        const bool cok = col < n;
+        double yn[4] = {0, 0, 0, 0}, dn[4] = {0, 0, 0, 0};
+        if (J + 1 < nblk) load_block(J + 1, yn, dn);
         // acc = sum_{K<J} Q[:,K] U[K,J]: four independent accumulator chains per
-        // k-step of 16, U loads software-pipelined one step ahead.
+        // k-step of 16; the U column is prefetched PF steps ahead through a
+        // register ring (the loop is unrolled by PF so the ring index is static).
+        constexpr int PF = 4;
         dh_d4 acc0 = {0, 0, 0, 0}, acc1 = acc0, acc2 = acc0, acc3 = acc0;
         const double* Ucol = U + (size_t)(cok ? col : 0) * ldu;
         const double cm = cok ? 1.0 : 0.0;
-        double b0 = 0, b1 = 0, b2 = 0, b3 = 0;
-        if (c0 > 0) {
-            b0 = Ucol[g];
-            b1 = Ucol[4 + g];
-            b2 = Ucol[8 + g];
-            b3 = Ucol[12 + g];
-        }
-        for (int k = 0; k < c0; k += 16) {
-            double c0v = b0, c1v = b1, c2v = b2, c3v = b3;
-            if (k + 16 < c0) {
-                b0 = Ucol[k + 16 + g];
-                b1 = Ucol[k + 20 + g];
-                b2 = Ucol[k + 24 + g];
-                b3 = Ucol[k + 28 + g];
-            }
+        double bq[PF][4];
+#pragma unroll
+        for (int p = 0; p < PF; ++p)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) bq[p][q] = (16 * p < c0) ? Ucol[16 * p + 4 * q + g] : 0.0;
+        auto step = [&](int k, double (&b)[4]) {
             const double* qa = Qs + li * pitch + k + g;  // Q[r0+li][k+g+4s]
-            acc0 = Mfma<double>::op(qa[0], c0v * cm, acc0);
-            acc1 = Mfma<double>::op(qa[4], c1v * cm, acc1);
-            acc2 = Mfma<double>::op(qa[8], c2v * cm, acc2);
-            acc3 = Mfma<double>::op(qa[12], c3v * cm, acc3);
+            acc0 = Mfma<double>::op(qa[0], b[0] * cm, acc0);
+            acc1 = Mfma<double>::op(qa[4], b[1] * cm, acc1);
+            acc2 = Mfma<double>::op(qa[8], b[2] * cm, acc2);
+            acc3 = Mfma<double>::op(qa[12], b[3] * cm, acc3);
+        };
+        int k = 0;
+        for (; k + 16 * PF <= c0; k += 16 * PF) {
+#pragma unroll
+            for (int p = 0; p < PF; ++p) {
+                double b[4] = {bq[p][0], bq[p][1], bq[p][2], bq[p][3]};
+                const int kn = k + 16 * (p + PF);
+                if (kn < c0)  // wave-uniform
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) bq[p][q] = Ucol[kn + 4 * q + g];
+                step(k + 16 * p, b);
+            }
         }
+#pragma unroll
+        for (int p = 0; p < PF; ++p)
+            if (k + 16 * p < c0) step(k + 16 * p, bq[p]);
         dh_d4 acc = (acc0 + acc1) + (acc2 + acc3);
         // T = Y_J - acc; accumulator r of lane L is row g + 4r, column li.
 #pragma unroll
@@ -331,6 +351,11 @@ __device__ __forceinline__ void trsm_rows_body(int blk, int m, int n, double* __
             if (row < m && cok) Y[row + (size_t)col * ldy] = q[r];
         }
         wave_lds_sync();
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            yv[r] = yn[r];
+            dv[r] = dn[r];
+        }
     }
 }
 
