@@ -5,10 +5,12 @@ Gaussian-mixture workload (16 centres in [-1,1]^128, sigma = 0.25, gamma = 0.5/d
 Y = K*Omega, CholeskyQR2, two power iterations, W = K*Q (B = W^T), inputs
 resident in HBM.  GFLOP/s counts F_MFMA + F_orth:
     F_MFMA = (q+2) * (2 n^2 d + 2 n^2 l),  F_orth = (q+1) * (6 n l^2 + 2/3 l^3)
-Besides the headline (BF16 K-passes) the line carries the FP64 solve, the
-FP16 K-passes, the 16384 core solve, the batched workload (96 problems,
-one graph), the largest workload (n = 262144, d = 256, k = 512, FP8
-distance) and the CPU baseline.
+The headline is the mixed path whose error matches FP64 (||K-QB||/||K||
+within 1.1x): FP16 K-passes on power-of-two scaled coordinates with FP32
+accumulation and FP64 CholeskyQR2.  Besides it the line carries the FP64
+solve, the BF16 K-passes (faster, less accurate), the 16384 core solve, the
+batched workload (96 problems, one graph), the largest workload (n = 262144,
+d = 256, k = 512, FP8 distance) and the CPU baseline.
 Prints ONE JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
 """
 from __future__ import annotations
@@ -259,17 +261,20 @@ def main(argv=None) -> int:
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps
 
+    # The headline ("mixed") is the FP16 mode: it is the mixed path whose
+    # error matches the FP64 one (ratio <= 1.1); BF16 K-passes are faster
+    # but about 1.13x less accurate on this workload and are reported beside.
     res = {}
     for name, prec, steps, warm in (("fp64", dh.DH_PREC_F64, a.steps_fp64, min(a.warmup, 1)),
-                                    ("f16", dh.DH_PREC_F16, a.steps, a.warmup),
-                                    ("mixed", dh.DH_PREC_BF16, a.steps, a.warmup)):
+                                    ("bf16", dh.DH_PREC_BF16, a.steps, a.warmup),
+                                    ("mixed", dh.DH_PREC_F16, a.steps, a.warmup)):
         s = timed(prec, steps, warm)
         r = {"s_per_solve": s, "gflops": (f_mfma + f_orth) / s / 1e9, "steps": steps}
         if not a.no_error:
             e2, k2 = h.lra_error(Xd, gamma, Q, W)
             r["rel_err"] = float(np.sqrt(max(e2, 0.0) / k2))
         res[name] = r
-    mixed, fp64, f16 = res["mixed"], res["fp64"], res["f16"]
+    mixed, fp64, bf16 = res["mixed"], res["fp64"], res["bf16"]
 
     # Dominant kernel of each path: the fused K*Omega projection kernel alone,
     # timed by the library with HIP events on the stream it is launched on.
@@ -286,14 +291,22 @@ def main(argv=None) -> int:
         h.timing(False)
         return ms / 1e3 / launches
 
+    t_f16 = kernel_time(lambda: h.rbf_project_f16(Xd, Xd, Omd, gamma, Yd), 10)
     t_bf16 = kernel_time(lambda: h.rbf_project_bf16(Xd, Xd, Omd, gamma, Yd), 10)
     t_f64 = kernel_time(lambda: h.rbf_project_f64(Xd, Xd, Omd, gamma, Yd), 2)
-    roofline = {"bound": "mfma", "achieved": pass_flops / t_bf16 / 1e12,
+    # Dense FP16 and BF16 MFMA share the 2.5 PFLOP/s peak.
+    roofline = {"bound": "mfma", "achieved": pass_flops / t_f16 / 1e12,
                 "peak": PEAK_BF16_MFMA / 1e12, "unit": "TFLOP/s",
-                "frac": pass_flops / t_bf16 / PEAK_BF16_MFMA,
-                "traffic": measured_traffic(ROOT / "profiles" / "rbf_project_bf16_n65536_d128_l272.md"),
-                "kernel": "rbf_project_bf16_kernel<4,17,8>", "kernel_ms": t_bf16 * 1e3,
+                "frac": pass_flops / t_f16 / PEAK_BF16_MFMA,
+                "traffic": measured_traffic(ROOT / "profiles" / "rbf_project_f16_n65536_d128_l272.md"),
+                "kernel": "rbf_project_bf16_kernel<4,17,8,F16>", "kernel_ms": t_f16 * 1e3,
                 "flops_per_launch": pass_flops}
+    bf16["roofline"] = {"bound": "mfma", "achieved": pass_flops / t_bf16 / 1e12,
+                        "peak": PEAK_BF16_MFMA / 1e12, "unit": "TFLOP/s",
+                        "frac": pass_flops / t_bf16 / PEAK_BF16_MFMA,
+                        "traffic": measured_traffic(
+                            ROOT / "profiles" / "rbf_project_bf16_n65536_d128_l272.md"),
+                        "kernel": "rbf_project_bf16_kernel<4,17,8>", "kernel_ms": t_bf16 * 1e3}
     fp64["roofline"] = {"bound": "mfma", "achieved": pass_flops / t_f64 / 1e12,
                         "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s",
                         "frac": pass_flops / t_f64 / PEAK_FP64_MFMA,
@@ -303,18 +316,19 @@ def main(argv=None) -> int:
         "metric": "RBF-LRA GFLOP/s & s/solve, mixed vs FP64 on 1 MI355X at matched ||K-QB||/||K||",
         "value": mixed["gflops"], "unit": "GFLOP/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": mixed["s_per_solve"] * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f16", "data": "synthetic",
         "config": {"workload": "rbf_lra_gaussian_mixture", "n": n, "d": d, "k": a.k, "p": a.p,
                    "q": q, "l": l, "gamma": gamma, "centres": 16, "sigma": 0.25,
-                   "precision": "bf16 K-passes (fp32 accumulate), fp64 CholeskyQR2"},
-        "mixed": mixed, "fp64": fp64, "f16": f16,
+                   "precision": "fp16 K-passes (scaled coordinates, fp32 accumulate), "
+                                "fp64 CholeskyQR2"},
+        "mixed": mixed, "fp64": fp64, "bf16": bf16,
         "speedup_vs_fp64": fp64["s_per_solve"] / mixed["s_per_solve"],
         "roofline": roofline,
     }
     if "rel_err" in mixed:
         line["err_ratio_vs_fp64"] = mixed["rel_err"] / fp64["rel_err"]
-        f16["err_ratio_vs_fp64"] = f16["rel_err"] / fp64["rel_err"]
-    f16["speedup_vs_fp64"] = fp64["s_per_solve"] / f16["s_per_solve"]
+        bf16["err_ratio_vs_fp64"] = bf16["rel_err"] / fp64["rel_err"]
+    bf16["speedup_vs_fp64"] = fp64["s_per_solve"] / bf16["s_per_solve"]
     if a.batched > 0:
         line["batched"] = batched_workload(h, a.batched, 3, not a.no_error)
     if a.largest > 0:

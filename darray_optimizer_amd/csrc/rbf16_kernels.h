@@ -248,7 +248,12 @@ __device__ __forceinline__ void project_rows(
         }
     };
     // K = clamp(exp2(c2 s + a_i + b_j), 0, 1) as the projection's A operand.
+    // FP16 values are converted in pairs from two-element vectors, the form
+    // the compiler turns into v_cvt_pk_f16_f32 (with the [0, 1] clamp folded
+    // into v_exp_f32); element-wise FP16 conversions cost two more
+    // instructions per pair.  BF16 pairs are packed either way.
     auto epilogue = [&](int buf, const f32x4 (&s)[2][RB], bf16x8 (&kb)[RB]) {
+        float ev[RB][8];
 #pragma unroll
         for (int jb = 0; jb < 2; ++jb)
 #pragma unroll
@@ -259,9 +264,27 @@ __device__ __forceinline__ void project_rows(
                 for (int ib = 0; ib < RB; ++ib) {
                     float cc = D8 ? crow[ib] * sj : crow[ib];
                     float e = __builtin_amdgcn_exp2f(fmaf(cc, s[jb][ib][r], arow[ib] + bv));
-                    kb[ib][4 * jb + r] = to16<F16>(__builtin_amdgcn_fmed3f(e, 0.f, 1.f));
+                    ev[ib][4 * jb + r] = __builtin_amdgcn_fmed3f(e, 0.f, 1.f);
                 }
             }
+#pragma unroll
+        for (int ib = 0; ib < RB; ++ib) {
+            if constexpr (F16) {
+                typedef float f32x2 __attribute__((ext_vector_type(2)));
+                typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+                typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+                u32x4 w;
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    w[e] = __builtin_bit_cast(
+                        unsigned,
+                        __builtin_convertvector((f32x2{ev[ib][2 * e], ev[ib][2 * e + 1]}), f16x2));
+                kb[ib] = __builtin_bit_cast(bf16x8, w);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) kb[ib][e] = to16<false>(ev[ib][e]);
+            }
+        }
     };
     // Projection B operands go through a ring of PF registers: the read of
     // column block cb + PF is issued before the MFMAs of block cb.
