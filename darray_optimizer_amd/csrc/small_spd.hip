@@ -359,6 +359,112 @@ __device__ __forceinline__ void trsm_rows_body(int blk, int m, int n, double* __
     }
 }
 
+// Q = Y U^{-1} in place for n <= 16 * NBM (the panel widths of the range
+// finder up to l = 272), one wave per 16 rows, with the wave's solved blocks
+// kept in registers in the MFMA A-operand layout,
+//   qa[K][s] = Q[r0 + li][16 K + 4 s + g],
+// instead of an LDS panel, so LDS does not cap the waves per SIMD.  For each
+// block column J the workgroup stages U[0:16J, 16J:16J+16] in LDS once
+// (coalesced column reads; every wave then reads its B operands there
+// instead of 16 scattered columns from L2), and a 16 x 17 LDS tile per wave
+// turns each solved block from the accumulator layout (row g + 4r, column
+// li) into the operand layout.  Both block loops are unrolled (register
+// arrays need static indices); blocks past n are skipped by a uniform test.
+// Every wave of the workgroup must call this (workgroup barriers); rows
+// past m are neither read nor written.
+template <int NBM, int WAVES>
+__device__ __forceinline__ void trsm_rows_reg_body(int r0, int m, int n, double* __restrict__ Y,
+                                                   int ldy, const double* __restrict__ U,
+                                                   int ldu, const double* __restrict__ Dinv,
+                                                   double (*tile)[17], double (*sU)[17]) {
+    const int lane = threadIdx.x & 63, g = lane >> 4, li = lane & 15;
+    const int nblk = (n + 15) / 16;
+    double qa[NBM][4];
+#pragma unroll
+    for (int J = 0; J < NBM; ++J) {
+        if (J < nblk) {  // uniform over the workgroup
+            const int col = 16 * J + li;
+            const bool cok = col < n;
+            double yv[4], dv[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = r0 + g + 4 * r;
+                yv[r] = (row < m && cok) ? Y[row + (size_t)col * ldy] : 0.0;
+            }
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) dv[ks] = Dinv[256 * J + (4 * ks + g) + 16 * li];
+            if (J > 0) {
+                // sU[k][c] = U[k][16J + c] for k < 16J (zero past column n).
+                __syncthreads();  // the previous block column's readers are done
+                constexpr int NT_ = WAVES * 64;
+                for (int e = threadIdx.x; e < 256 * J; e += NT_) {
+                    const int k = e % (16 * J), c = e / (16 * J);
+                    const bool ok = 16 * J + c < n;
+                    const double v = U[k + (size_t)min(16 * J + c, n - 1) * ldu];
+                    sU[k][c] = ok ? v : 0.0;
+                }
+                __syncthreads();
+            }
+            // acc = sum_{K<J} Q_K U_KJ, one accumulator chain per k-slot s.
+            dh_d4 acc[4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+#pragma unroll
+            for (int K = 0; K < J; ++K)
+#pragma unroll
+                for (int s4 = 0; s4 < 4; ++s4)
+                    acc[s4] = Mfma<double>::op(qa[K][s4], sU[16 * K + 4 * s4 + g][li], acc[s4]);
+            const dh_d4 a = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+            // T = Y_J - acc (row g + 4r, column li) -> operand layout -> Q_J = T U_JJ^{-1}.
+#pragma unroll
+            for (int r = 0; r < 4; ++r) tile[g + 4 * r][li] = yv[r] - a[r];
+            wave_lds_sync();
+            dh_d4 q = {0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) q = Mfma<double>::op(tile[li][4 * ks + g], dv[ks], q);
+            wave_lds_sync();
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = r0 + g + 4 * r;
+                if (row < m && cok) Y[row + (size_t)col * ldy] = q[r];
+                tile[g + 4 * r][li] = q[r];
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) qa[J][s4] = tile[li][4 * s4 + g];
+            wave_lds_sync();
+        }
+    }
+}
+
+constexpr int kRegBlocks = 17;  // register-resident row solve for n <= 272
+
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void trsm_rows_reg_kernel(int m, int n,
+                                                                   double* __restrict__ Y,
+                                                                   int ldy,
+                                                                   const double* __restrict__ U,
+                                                                   int ldu,
+                                                                   const double* __restrict__ Dinv) {
+    __shared__ double tile[WAVES][16][17];
+    __shared__ double sU[16 * (kRegBlocks - 1)][17];
+    const int wave = threadIdx.x >> 6;
+    const int r0 = (blockIdx.x * WAVES + wave) * 16;
+    trsm_rows_reg_body<kRegBlocks, WAVES>(r0, m, n, Y, ldy, U, ldu, Dinv, tile[wave], sU);
+}
+
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void trsm_rows_reg_batched_kernel(
+    const DhSpdJob* __restrict__ jobs) {
+    __shared__ double tile[WAVES][16][17];
+    __shared__ double sU[16 * (kRegBlocks - 1)][17];
+    const DhSpdJob& j = jobs[blockIdx.y];
+    // Whole-workgroup exit: the body synchronises the workgroup.
+    if (blockIdx.x * WAVES * 16 >= (unsigned)j.m || *j.info != 0) return;
+    const int wave = threadIdx.x >> 6;
+    const int r0 = (blockIdx.x * WAVES + wave) * 16;
+    trsm_rows_reg_body<kRegBlocks, WAVES>(r0, j.m, j.n, j.Y, j.ldy, j.G, j.ldg, j.Dinv,
+                                          tile[wave], sU);
+}
+
 template <int WAVES>
 __global__ __launch_bounds__(WAVES * 64) void trsm_rows_kernel(int m, int n,
                                                                double* __restrict__ Y, int ldy,
@@ -411,6 +517,13 @@ int dh_cholqr_step_batched(dh_handle_t h, const DhSpdJob* jobs, int count, int m
     if (count == 0) return 0;
     chol_small_batched_kernel<<<count, NT, 0, h->stream>>>(jobs, shift ? 1 : 0, 1);
     DH_CHECK_LAUNCH();
+    if (max_n <= 16 * kRegBlocks) {
+        constexpr int WR = 4;
+        trsm_rows_reg_batched_kernel<WR>
+            <<<dim3(dh_cdiv(max_m, 16 * WR), count), WR * 64, 0, h->stream>>>(jobs);
+        DH_CHECK_LAUNCH();
+        return 0;
+    }
     int pitch = 16 * ((max_n + 15) / 16) + 1;
     size_t lds = sizeof(double) * 16 * (size_t)pitch * W;
     dim3 grid(dh_cdiv(max_m, 16 * W), count);
@@ -424,6 +537,13 @@ int dh_trsm_rows_launch(dh_handle_t h, int m, int n, double* Y, int ldy, const d
                         int ldu, const double* Dinv) {
     if (n > kMaxN) return dh_fail(DH_ERR_UNSUPPORTED, "trsm_rows: order %d > %d", n, kMaxN);
     if (m == 0 || n == 0) return 0;
+    if (n <= 16 * kRegBlocks) {
+        constexpr int WR = 4;
+        trsm_rows_reg_kernel<WR><<<dh_cdiv(m, 16 * WR), WR * 64, 0, h->stream>>>(m, n, Y, ldy, U,
+                                                                               ldu, Dinv);
+        DH_CHECK_LAUNCH();
+        return 0;
+    }
     constexpr int W = 2;
     int pitch = 16 * ((n + 15) / 16) + 1;
     size_t lds = sizeof(double) * 16 * (size_t)pitch * W;

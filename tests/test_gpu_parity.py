@@ -168,6 +168,19 @@ def test_trsm_f64(dev, m, side, trans):
     assert rel(dh.from_dev(Bd), ref) < 1e-12
 
 
+@pytest.mark.parametrize("side,trans,m,nrhs", [("L", "T", 700, 64), ("L", "N", 700, 64),
+                                                ("L", "T", 300, 600), ("L", "N", 130, 513)])
+def test_trsm_f64_block_sweep_and_recursion(dev, side, trans, m, nrhs):
+    """Side-L solves with up to 512 right-hand sides run as a sweep over 64-row
+    blocks; wider ones take the recursive halving.  Both against the oracle."""
+    rng = np.random.default_rng(m + nrhs)
+    U = np.triu(rng.standard_normal((m, m))) + 8 * np.eye(m)
+    B = rng.standard_normal((m, nrhs))
+    Bd = dev_mat(B)
+    dev.trsm(side, trans, "N", dev_mat(U), Bd)
+    assert rel(dh.from_dev(Bd), orc.trsm(side, trans, U, B)) < 1e-12
+
+
 def test_trsm_golden(dev):
     U = G["trsm_U"]
     for side, trans, Bk, ref in (("L", "T", "trsm_B", "trsm_LUT"), ("L", "N", "trsm_B", "trsm_LUN"),
