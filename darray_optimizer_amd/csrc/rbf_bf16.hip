@@ -18,9 +18,18 @@ __global__ __launch_bounds__(256) void colsum_partial_kernel(int n, int d, const
     int b = blockIdx.x;
     int i1 = min(n, (b + 1) * slice);
     for (int f = threadIdx.x; f < d; f += 256) {
-        double s = 0.0;
-        for (int i = b * slice; i < i1; ++i) s += X[f + (size_t)i * ldx];
-        partial[(size_t)b * d + f] = s;
+        // Four independent sums so that the loads of consecutive points
+        // overlap (fixed order: deterministic).
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        int i = b * slice;
+        for (; i + 4 <= i1; i += 4) {
+            s0 += X[f + (size_t)i * ldx];
+            s1 += X[f + (size_t)(i + 1) * ldx];
+            s2 += X[f + (size_t)(i + 2) * ldx];
+            s3 += X[f + (size_t)(i + 3) * ldx];
+        }
+        for (; i < i1; ++i) s0 += X[f + (size_t)i * ldx];
+        partial[(size_t)b * d + f] = (s0 + s1) + (s2 + s3);
     }
 }
 
@@ -28,9 +37,16 @@ __global__ __launch_bounds__(256) void colmean_final_kernel(int n, int d, int nb
                                                             const double* __restrict__ partial,
                                                             double* __restrict__ mean) {
     for (int f = blockIdx.x * 256 + threadIdx.x; f < d; f += gridDim.x * 256) {
-        double s = 0.0;
-        for (int b = 0; b < nb; ++b) s += partial[(size_t)b * d + f];
-        mean[f] = n > 0 ? s / n : 0.0;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        int b = 0;
+        for (; b + 4 <= nb; b += 4) {
+            s0 += partial[(size_t)b * d + f];
+            s1 += partial[(size_t)(b + 1) * d + f];
+            s2 += partial[(size_t)(b + 2) * d + f];
+            s3 += partial[(size_t)(b + 3) * d + f];
+        }
+        for (; b < nb; ++b) s0 += partial[(size_t)b * d + f];
+        mean[f] = n > 0 ? ((s0 + s1) + (s2 + s3)) / n : 0.0;
     }
 }
 
