@@ -260,6 +260,32 @@ def test_potrf_reports_first_bad_minor(dev, bad):
     assert dev.potrf(Ad, 64) == bad + 1
 
 
+def test_potrf_lookahead_orders_with_the_handle_stream(dev):
+    """The look-ahead runs part of each trailing update on a second stream:
+    it must start after work queued before the call on the handle's stream
+    (here the copy that fills A) and finish before the call returns."""
+    import torch
+    rng = np.random.default_rng(16)
+    n = 1500
+    M = rng.standard_normal((n, n))
+    A = M.T @ M / n + np.eye(n)
+    src = dev_mat(A)
+    side = torch.cuda.Stream()
+    h = dh.Handle(0, stream=side)
+    try:
+        with torch.cuda.stream(side):
+            Ad = torch.full_like(src, float("nan"))
+            torch.cuda._sleep(2_000_000)  # keep the copy below queued behind other work
+            Ad.copy_(src)
+        assert h.potrf(Ad, 256) == 0
+        side.synchronize()
+        Uo, info = orc.potrf_upper(A, 256)
+        assert info == 0
+        assert rel(np.triu(dh.from_dev(Ad)), np.triu(Uo)) < 1e-12
+    finally:
+        h.close()
+
+
 # ------------------------------------------------------------- CholeskyQR2
 
 def test_cholqr2_golden(dev):
