@@ -172,22 +172,41 @@ __global__ __launch_bounds__(256) void rbf_project_f64_kernel(
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb) acc[cb] = dh_d4{0, 0, 0, 0};
 
+    // Staging of a chunk: every load is issued before any LDS store, from
+    // addresses clamped into the operands (a load under a branch gets a wait
+    // of its own and the chunk's ~25 loads per thread would run one after
+    // another).  Thread tid always stages point jj = tid % JB of the Omega
+    // block, so its label is one load per chunk.
+    constexpr int NXL = JB * FD / 256, NOL = JB * NCB * 16 / 256;
+    static_assert(JB * FD % 256 == 0 && JB * NCB * 16 % 256 == 0, "staging split");
     for (int j0 = 0; j0 < n; j0 += JB) {
-        __syncthreads();
-        for (int e = tid; e < JB * FD; e += 256) {
-            int jj = e / FD, f = e % FD, j = j0 + jj;
-            sX[f][jj] = (j < n && f < d) ? Xj[f + (size_t)j * ldxj] : 0.0;
+        double px[NXL], po[NOL];
+#pragma unroll
+        for (int u = 0; u < NXL; ++u) {
+            const int e = tid + 256 * u, jj = e / FD, f = e % FD;
+            px[u] = Xj[min(f, d - 1) + (size_t)min(j0 + jj, n - 1) * ldxj];
         }
-        for (int e = tid; e < JB * NCB * 16; e += 256) {
-            int jj = e % JB, cl = e / JB, j = j0 + jj, c = c0 + cl;
-            double v = 0.0;
-            if (j < n && c < l) {
-                v = O[j + (size_t)c * ldo];
-                if (yj) v *= yj[j];
-            }
-            sO[jj >> 2][cl >> 4][(jj & 3) * 16 + (cl & 15)] = v;
+        const int jo = j0 + (tid % JB);
+        const double yv = yj ? yj[min(jo, n - 1)] : 1.0;
+#pragma unroll
+        for (int u = 0; u < NOL; ++u) {
+            const int e = tid + 256 * u, cl = e / JB;
+            po[u] = O[min(jo, n - 1) + (size_t)min(c0 + cl, l - 1) * ldo];
         }
-        if (tid < JB) sN[tid] = (j0 + tid < n) ? nj[j0 + tid] : 0.0;
+        const double pn = nj[min(j0 + (tid % JB), n - 1)];
+        __syncthreads();  // the previous chunk's readers are done
+#pragma unroll
+        for (int u = 0; u < NXL; ++u) {
+            const int e = tid + 256 * u, jj = e / FD, f = e % FD;
+            sX[f][jj] = (j0 + jj < n && f < d) ? px[u] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < NOL; ++u) {
+            const int e = tid + 256 * u, jj = e % JB, cl = e / JB;
+            sO[jj >> 2][cl >> 4][(jj & 3) * 16 + (cl & 15)] =
+                (jo < n && c0 + cl < l) ? po[u] * yv : 0.0;
+        }
+        if (tid < JB) sN[tid] = (j0 + tid < n) ? pn : 0.0;
         __syncthreads();
 
         dh_d4 s = dh_d4{0, 0, 0, 0};
