@@ -1,7 +1,8 @@
 // The k x k step of CholeskyQR on gfx950, and the row-parallel triangular
 // solve that applies its factor to the n x k panel.
 //
-// chol_small_kernel (one workgroup, order n <= 544): G (upper triangle read)
+// chol_reg_kernel (one workgroup, order n <= 272: the matrix held in
+// registers) and chol_small_kernel (order n <= 544): G (upper triangle read)
 // is overwritten by U with G = U^T U; the strict lower triangle is untouched.
 // Right-looking in 16-column panels as cholesky_factorize (lapack_like.h:17-52):
 // factor the diagonal block (one wave, column per lane), solve the row panel
@@ -45,53 +46,63 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// Breakdown tolerance and CholeskyQR3 shift of G from its diagonal (every
+// thread of the workgroup gets both).
+__device__ __forceinline__ void chol_scales(int n, const double* __restrict__ G, int ldg,
+                                            int m_rows, int shift, int rel_check, double& tol,
+                                            double& sh) {
+    __shared__ double red[2][NT / 64];
+    __shared__ double s_tol, s_shift;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int NW = NT / 64;
+    double mx = 0.0, tr = 0.0;
+    for (int k = tid; k < n; k += NT) {
+        double v = G[k + (size_t)k * ldg];
+        mx = fmax(mx, v);
+        tr += v;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mx = fmax(mx, __shfl_xor(mx, o));
+        tr += __shfl_xor(tr, o);
+    }
+    if (lane == 0) {
+        red[0][wave] = mx;
+        red[1][wave] = tr;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double m2 = 0.0, t2 = 0.0;
+        for (int w = 0; w < NW; ++w) {
+            m2 = fmax(m2, red[0][w]);
+            t2 += red[1][w];
+        }
+        s_tol = rel_check ? (double)n * 2.220446049250313e-16 * m2 : 0.0;
+        s_shift = shift ? 11.0 * ((double)m_rows * n + (double)n * (n + 1)) *
+                              1.1102230246251565e-16 * t2
+                        : 0.0;
+    }
+    __syncthreads();
+    tol = s_tol;
+    sh = s_shift;
+}
+
 __device__ __forceinline__ void chol_small_body(int n, double* __restrict__ G, int ldg,
                                                 double* __restrict__ Dinv, int m_rows, int shift,
                                                 int rel_check, int* __restrict__ info) {
     __shared__ double sD[NB][NB + 1];            // diagonal block, sD[j][i] = A(k0+i, k0+j)
     __shared__ __align__(16) double sP[NB][PW];  // row panel U12, sP[kk][c]
-    __shared__ double red[2][NT / 64];
-    __shared__ double s_tol, s_shift;
     __shared__ int s_info;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, li = lane & 15;  // MFMA lane group and index
     constexpr int NW = NT / 64;
 
-    {
-        double mx = 0.0, tr = 0.0;
-        for (int k = tid; k < n; k += NT) {
-            double v = G[k + (size_t)k * ldg];
-            mx = fmax(mx, v);
-            tr += v;
-        }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            mx = fmax(mx, __shfl_xor(mx, o));
-            tr += __shfl_xor(tr, o);
-        }
-        if (lane == 0) {
-            red[0][wave] = mx;
-            red[1][wave] = tr;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            double m2 = 0.0, t2 = 0.0;
-            for (int w = 0; w < NW; ++w) {
-                m2 = fmax(m2, red[0][w]);
-                t2 += red[1][w];
-            }
-            s_tol = rel_check ? (double)n * 2.220446049250313e-16 * m2 : 0.0;
-            s_shift = shift ? 11.0 * ((double)m_rows * n + (double)n * (n + 1)) *
-                                  1.1102230246251565e-16 * t2
-                            : 0.0;
-            s_info = 0;
-        }
-        __syncthreads();
-        if (shift)
-            for (int k = tid; k < n; k += NT) G[k + (size_t)k * ldg] += s_shift;
-        __syncthreads();
-    }
-    const double tol = s_tol;
+    double tol, sh;
+    chol_scales(n, G, ldg, m_rows, shift, rel_check, tol, sh);
+    if (tid == 0) s_info = 0;
+    if (shift)
+        for (int k = tid; k < n; k += NT) G[k + (size_t)k * ldg] += sh;
+    __syncthreads();
 
     for (int k0 = 0; k0 < n; k0 += NB) {
         const int nb = min(NB, n - k0), w = n - k0 - nb;
@@ -486,6 +497,215 @@ __global__ __launch_bounds__(WAVES * 64) void trsm_rows_batched_kernel(
                           16 * ((j.n + 15) / 16) + 1);
 }
 
+// Register-resident Cholesky for orders n <= 16 * kRegBlocks (= 272), same
+// contract as chol_small_body.  The upper triangle of the order-16*nb matrix
+// (nb = ceil(n/16); rows and columns past n hold the identity, so they factor
+// to the identity and skip the pivot test) lives in registers as 16 x 16
+// blocks in FP64 MFMA accumulator layout (lane: rows g + 4r, column li).
+// Blocks (bi, bj >= bi) are numbered row by row and dealt to the 8 waves
+// round robin, at most kCholSlots per wave.  Per 16-column panel k:
+//   A  the owner of block (k, k) factors it in-wave (column per lane, rows
+//      broadcast by readlane), inverts the factor by back substitution, and
+//      writes U_kk to G, its inverse to Dinv and both to LDS;
+//   B  the owners of blocks (k, j > k) form U_kj = U_kk^{-T} A_kj (4 MFMAs,
+//      A_kj's accumulator rows are the B operand) into the LDS panel, which
+//      the workgroup then copies to G;
+//   C  the owners of blocks (i, j), k < i <= j, update A_ij -= U_ki^T U_kj
+//      (4 MFMAs from the LDS panel).  The owner of block (k+1, k+1) updates
+//      it first and runs step A of panel k+1 before its other updates, so
+//      that factorisation overlaps the rest of the trailing update.
+// Two workgroup barriers per panel and no global loads after the first.
+constexpr int kCholWaves = NT / 64;
+constexpr int kCholSlots = (kRegBlocks * (kRegBlocks + 1) / 2 + kCholWaves - 1) / kCholWaves;
+
+__device__ __forceinline__ void chol_reg_body(int n, double* __restrict__ G, int ldg,
+                                              double* __restrict__ Dinv, int m_rows, int shift,
+                                              int rel_check, int* __restrict__ info) {
+    __shared__ double sD[NB][NB + 1];  // diagonal block, sD[j][i] = A(i, j)
+    __shared__ double sI[NB][NB + 1];  // its inverse factor, sI[j][i] = U_kk^{-1}(i, j)
+    __shared__ __align__(16) double sP[NB][16 * (kRegBlocks - 1) + 4];  // U_k,k+1..nb-1
+    __shared__ int s_info;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int g = lane >> 4, li = lane & 15;
+
+    double tol, sh;
+    chol_scales(n, G, ldg, m_rows, shift, rel_check, tol, sh);
+    if (tid == 0) s_info = 0;
+    const int nbk = (n + 15) / 16;
+
+    int bI[kCholSlots], bJ[kCholSlots];  // block of each slot; bI == nbk: unused
+#pragma unroll
+    for (int s = 0; s < kCholSlots; ++s) {
+        int rem = s * kCholWaves + wave, bi = 0;
+        while (bi < nbk && rem >= nbk - bi) {
+            rem -= nbk - bi;
+            ++bi;
+        }
+        bI[s] = bi;
+        bJ[s] = bi + rem;
+    }
+    // Load in groups of four slots: the group's 16 loads are issued together
+    // (clamped addresses, no branches), then padding, the zero lower triangle
+    // of diagonal blocks and the shift are applied by selects.
+    dh_d4 a[kCholSlots];
+#pragma unroll
+    for (int s0 = 0; s0 < kCholSlots; s0 += 4) {
+        double v[4][4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int s = min(s0 + j, kCholSlots - 1);
+            const int cc = min(16 * bJ[s] + li, n - 1);
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                v[j][r] = G[min(16 * bI[s] + g + 4 * r, n - 1) + (size_t)cc * ldg];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int s = s0 + j;
+            if (s < kCholSlots) {
+                const int c = 16 * bJ[s] + li;
+                const bool dblk = bI[s] == bJ[s];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = 16 * bI[s] + g + 4 * r;
+                    double e = (row < n && c < n) ? v[j][r] : (row == c ? 1.0 : 0.0);
+                    e = (dblk && row > c) ? 0.0 : e;
+                    e = (dblk && row == c && c < n) ? e + sh : e;
+                    a[s][r] = e;
+                }
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);  // keep the groups apart (register pressure)
+    }
+
+    // Step A for panel k, by the wave holding block (k, k) in slot sd.
+    auto factor_diag = [&](int k, int sd) {
+#pragma unroll
+        for (int s = 0; s < kCholSlots; ++s)
+            if (s == sd)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sD[li][g + 4 * r] = a[s][r];
+        wave_lds_sync();
+        // Lane j holds column j of the block (lanes 16..63 a copy of lane
+        // j & 15, never read).  Entries below the diagonal are not needed and
+        // take whatever the unmasked updates leave there.
+        double u[NB];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) u[i] = sD[li][i];
+        const int nreal = n - 16 * k;
+        int bad = 0;
+#pragma unroll
+        for (int kk = 0; kk < NB; ++kk) {
+            if (!bad) {
+                const double piv = readlane_d(u[kk], kk);
+                if (kk < nreal && !(piv > tol)) {
+                    bad = kk + 1;
+                } else {
+                    // 1/sqrt(piv): hardware estimate and two Newton steps.
+                    const double hp = 0.5 * piv;
+                    double y = __builtin_amdgcn_rsq(piv);
+                    y = fma(y, fma(-hp * y, y, 0.5), y);
+                    y = fma(y, fma(-hp * y, y, 0.5), y);
+                    const double ukj = (lane == kk) ? piv * y : u[kk] * y;
+                    u[kk] = ukj;
+#pragma unroll
+                    for (int i = kk + 1; i < NB; ++i) u[i] -= readlane_d(ukj, i) * ukj;
+                }
+            }
+        }
+        const int c = 16 * k + lane;
+        if (lane < NB && c < n)
+#pragma unroll
+            for (int i = 0; i < NB; ++i)
+                if (i <= lane) G[(16 * k + i) + (size_t)c * ldg] = u[i];
+        // Column j of U_kk^{-1} by back substitution, in place: step i reads
+        // row i of U (u[i] of lanes q > i) and then overwrites u[i] with x[i].
+        // Lane i holds 1 / U(i, i), formed once.
+        double dg = 1.0;
+#pragma unroll
+        for (int i = 0; i < NB; ++i)
+            if (i == lane) dg = u[i];
+        const double rdg = 1.0 / dg;
+#pragma unroll
+        for (int i = NB - 1; i >= 0; --i) {
+            double t = (i == lane) ? 1.0 : 0.0;
+#pragma unroll
+            for (int q = i + 1; q < NB; ++q) t -= readlane_d(u[i], q) * u[q];
+            u[i] = t * readlane_d(rdg, i);
+        }
+        if (lane < NB)
+#pragma unroll
+            for (int i = 0; i < NB; ++i) {
+                sI[lane][i] = u[i];
+                Dinv[256 * k + i + 16 * lane] = (i <= lane && c < n) ? u[i] : 0.0;
+            }
+        if (lane == 0 && bad) s_info = 16 * k + bad;
+    };
+
+    __syncthreads();  // s_info initialised
+    if (wave == 0) factor_diag(0, 0);  // block (0, 0) is block 0: wave 0, slot 0
+    for (int k = 0;; ++k) {
+        __syncthreads();  // U_kk, its inverse and s_info ready
+        if (s_info || k + 1 == nbk) break;
+        // ---- B: row panel U_kj = U_kk^{-T} A_kj.
+#pragma unroll
+        for (int s = 0; s < kCholSlots; ++s) {
+            if (bI[s] == k && bJ[s] > k) {
+                dh_d4 acc = {0, 0, 0, 0};
+#pragma unroll
+                for (int s4 = 0; s4 < 4; ++s4)
+                    acc = Mfma<double>::op(sI[li][4 * s4 + g], a[s][s4], acc);
+                const int pc = 16 * (bJ[s] - k - 1) + li;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sP[g + 4 * r][pc] = acc[r];
+            }
+        }
+        __syncthreads();  // panel k in sP
+        {  // the panel to G, 16-row runs per column
+            const int cols = min(n, 16 * nbk) - 16 * (k + 1);
+            for (int e = tid; e < 16 * cols; e += NT) {
+                const int r = e & 15, cl = e >> 4;
+                G[(16 * k + r) + (size_t)(16 * (k + 1) + cl) * ldg] = sP[r][cl];
+            }
+        }
+        // ---- C: trailing update A_ij -= U_ki^T U_kj, block (k+1, k+1)
+        // and step A of panel k+1 first on the wave that holds it.
+        const int kd = (k + 1) * nbk - (k + 1) * k / 2;  // number of block (k+1, k+1)
+        const int sd = (wave == kd % kCholWaves) ? kd / kCholWaves : -1;
+        auto update = [&](int s) {
+            const int pi = 16 * (bI[s] - k - 1) + li, pj = 16 * (bJ[s] - k - 1) + li;
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4)
+                a[s] = Mfma<double>::op(-sP[4 * s4 + g][pi], sP[4 * s4 + g][pj], a[s]);
+        };
+        if (sd >= 0) {
+#pragma unroll
+            for (int s = 0; s < kCholSlots; ++s)
+                if (s == sd) update(s);
+            factor_diag(k + 1, sd);
+        }
+#pragma unroll
+        for (int s = 0; s < kCholSlots; ++s)
+            if (bI[s] > k && bI[s] < nbk && s != sd) update(s);
+    }
+    if (tid == 0) *info = s_info;
+}
+
+__global__ __launch_bounds__(NT) void chol_reg_kernel(int n, double* __restrict__ G, int ldg,
+                                                      double* __restrict__ Dinv, int m_rows,
+                                                      int shift, int rel_check,
+                                                      int* __restrict__ info) {
+    chol_reg_body(n, G, ldg, Dinv, m_rows, shift, rel_check, info);
+}
+
+__global__ __launch_bounds__(NT) void chol_reg_batched_kernel(const DhSpdJob* __restrict__ jobs,
+                                                              int shift, int rel_check) {
+    const DhSpdJob& j = jobs[blockIdx.x];
+    chol_reg_body(j.n, j.G, j.ldg, j.Dinv, j.m, shift, rel_check, j.info);
+    if (threadIdx.x == 0 && *j.info != 0) *j.sticky = 1;
+}
+
 }  // namespace
 
 int dh_chol_small_max_order() { return kMaxN; }
@@ -495,8 +715,12 @@ int dh_chol_small_max_order() { return kMaxN; }
 int dh_chol_small_launch(dh_handle_t h, int n, double* G, int ldg, double* Dinv, int m_rows,
                          bool shift, bool rel_check, int* info_dev) {
     if (n > kMaxN) return dh_fail(DH_ERR_UNSUPPORTED, "chol_small: order %d > %d", n, kMaxN);
-    chol_small_kernel<<<1, NT, 0, h->stream>>>(n, G, ldg, Dinv, m_rows, shift ? 1 : 0,
-                                               rel_check ? 1 : 0, info_dev);
+    if (n <= 16 * kRegBlocks)
+        chol_reg_kernel<<<1, NT, 0, h->stream>>>(n, G, ldg, Dinv, m_rows, shift ? 1 : 0,
+                                                 rel_check ? 1 : 0, info_dev);
+    else
+        chol_small_kernel<<<1, NT, 0, h->stream>>>(n, G, ldg, Dinv, m_rows, shift ? 1 : 0,
+                                                   rel_check ? 1 : 0, info_dev);
     DH_CHECK_LAUNCH();
     return 0;
 }
@@ -515,7 +739,10 @@ int dh_cholqr_step_batched(dh_handle_t h, const DhSpdJob* jobs, int count, int m
         lds_attr_set = true;
     }
     if (count == 0) return 0;
-    chol_small_batched_kernel<<<count, NT, 0, h->stream>>>(jobs, shift ? 1 : 0, 1);
+    if (max_n <= 16 * kRegBlocks)
+        chol_reg_batched_kernel<<<count, NT, 0, h->stream>>>(jobs, shift ? 1 : 0, 1);
+    else
+        chol_small_batched_kernel<<<count, NT, 0, h->stream>>>(jobs, shift ? 1 : 0, 1);
     DH_CHECK_LAUNCH();
     if (max_n <= 16 * kRegBlocks) {
         constexpr int WR = 4;

@@ -296,9 +296,11 @@ def test_cholqr2_golden(dev):
     assert rel(dh.from_dev(R), G["qr_R"]) < 1e-12
 
 
-@pytest.mark.parametrize("m,n", [(20000, 138), (6000, 544), (4000, 600)])
+@pytest.mark.parametrize("m,n", [(20000, 138), (3000, 1), (3000, 257), (5000, 272), (6000, 544),
+                                 (4000, 600)])
 def test_cholqr2_tall(dev, m, n):
-    # n <= 544: one-launch Cholesky + inverse per sweep; n = 600: blocked fallback.
+    # n <= 272: register-resident Cholesky (257: one real column in the last
+    # 16-block); n <= 544: one-workgroup Cholesky; n = 600: blocked fallback.
     import torch
     rng = np.random.default_rng(16)
     Y = rng.standard_normal((m, n)) @ np.diag(np.logspace(0, -4, n))
@@ -313,12 +315,13 @@ def test_cholqr2_tall(dev, m, n):
     assert info == 0 and rel(Rh, Ro) < 1e-10
 
 
-@pytest.mark.parametrize("n", [10, 600])
+@pytest.mark.parametrize("n", [10, 250, 400, 600])
 def test_cholqr2_breakdown(dev, n):
-    # Exactly dependent columns: every CholeskyQR variant must report breakdown.
+    # Exactly dependent columns: every CholeskyQR variant must report breakdown
+    # (for n = 250 and 400 in a late panel).
     rng = np.random.default_rng(17)
     Y = rng.standard_normal((2000, n))
-    Y[:, 7] = Y[:, 2]
+    Y[:, (3 * n) // 4] = Y[:, 2]
     assert dev.cholqr2(dev_mat(Y)) > 0
 
 
