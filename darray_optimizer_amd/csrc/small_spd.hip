@@ -498,32 +498,37 @@ __global__ __launch_bounds__(WAVES * 64) void trsm_rows_batched_kernel(
 }
 
 // Register-resident Cholesky for orders n <= 16 * kRegBlocks (= 272), same
-// contract as chol_small_body.  The upper triangle of the order-16*nb matrix
-// (nb = ceil(n/16); rows and columns past n hold the identity, so they factor
-// to the identity and skip the pivot test) lives in registers as 16 x 16
-// blocks in FP64 MFMA accumulator layout (lane: rows g + 4r, column li).
-// Blocks (bi, bj >= bi) are numbered row by row and dealt to the 8 waves
-// round robin, at most kCholSlots per wave.  Per 16-column panel k:
-//   A  the owner of block (k, k) factors it in-wave (column per lane, rows
-//      broadcast by readlane), inverts the factor by back substitution, and
-//      writes U_kk to G, its inverse to Dinv and both to LDS;
+// contract as chol_small_body.  The order-16*nb matrix (nb = ceil(n/16); rows
+// and columns past n hold the identity, so they factor to the identity and
+// skip the pivot test) is held as 16 x 16 blocks: the off-diagonal blocks
+// (bi, bj > bi) in registers, in FP64 MFMA accumulator layout (lane: rows
+// g + 4r, column li), numbered row by row and dealt to the 8 waves round
+// robin; the diagonal blocks in LDS (sG[i][col][row]), block i updated by
+// wave i % 8.  Per 16-column panel k:
+//   A  wave k % 8 factors block (k, k) (column per lane; rows of U broadcast
+//      through LDS), inverts the factor by back substitution, and writes
+//      U_kk to G, its inverse to Dinv and to LDS;
 //   B  the owners of blocks (k, j > k) form U_kj = U_kk^{-T} A_kj (4 MFMAs,
 //      A_kj's accumulator rows are the B operand) into the LDS panel, which
 //      the workgroup then copies to G;
-//   C  the owners of blocks (i, j), k < i <= j, update A_ij -= U_ki^T U_kj
-//      (4 MFMAs from the LDS panel).  The owner of block (k+1, k+1) updates
-//      it first and runs step A of panel k+1 before its other updates, so
-//      that factorisation overlaps the rest of the trailing update.
+//   C  A_ij -= U_ki^T U_kj for k < i <= j (4 MFMAs per block from the LDS
+//      panel).  Wave (k+1) % 8 updates block (k+1, k+1) first and runs step
+//      A of panel k+1 before its other updates, so that factorisation
+//      overlaps the rest of the trailing update.
 // Two workgroup barriers per panel and no global loads after the first.
 constexpr int kCholWaves = NT / 64;
-constexpr int kCholSlots = (kRegBlocks * (kRegBlocks + 1) / 2 + kCholWaves - 1) / kCholWaves;
+constexpr int kCholSlots =
+    (kRegBlocks * (kRegBlocks - 1) / 2 + kCholWaves - 1) / kCholWaves;  // 17
 
 __device__ __forceinline__ void chol_reg_body(int n, double* __restrict__ G, int ldg,
                                               double* __restrict__ Dinv, int m_rows, int shift,
                                               int rel_check, int* __restrict__ info) {
-    __shared__ double sD[NB][NB + 1];  // diagonal block, sD[j][i] = A(i, j)
-    __shared__ double sI[NB][NB + 1];  // its inverse factor, sI[j][i] = U_kk^{-1}(i, j)
+    __shared__ double sG[kRegBlocks][NB][NB + 1];  // diagonal blocks, sG[i][col][row]
+    __shared__ double sI[NB][NB + 1];  // inverse factor of panel k, sI[j][i] = U_kk^{-1}(i, j)
     __shared__ __align__(16) double sP[NB][16 * (kRegBlocks - 1) + 4];  // U_k,k+1..nb-1
+    __shared__ __align__(16) double sU[NB][NB];  // U_kk^T for the back substitution
+    __shared__ __align__(16) double sR[2][NB];   // row kk of U_kk
+    __shared__ double sRd[NB];                   // 1 / U_kk(i, i)
     __shared__ int s_info;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -538,16 +543,25 @@ __device__ __forceinline__ void chol_reg_body(int n, double* __restrict__ G, int
 #pragma unroll
     for (int s = 0; s < kCholSlots; ++s) {
         int rem = s * kCholWaves + wave, bi = 0;
-        while (bi < nbk && rem >= nbk - bi) {
-            rem -= nbk - bi;
+        while (bi < nbk && rem >= nbk - 1 - bi) {
+            rem -= nbk - 1 - bi;
             ++bi;
         }
         bI[s] = bi;
-        bJ[s] = bi + rem;
+        bJ[s] = bi + 1 + rem;
     }
-    // Load in groups of four slots: the group's 16 loads are issued together
-    // (clamped addresses, no branches), then padding, the zero lower triangle
-    // of diagonal blocks and the shift are applied by selects.
+    // Diagonal blocks to LDS: upper triangle, zero below, identity padding,
+    // shifted diagonal.
+    for (int e = tid; e < 256 * nbk; e += NT) {
+        const int i = e >> 8, row = e & 15, col = (e >> 4) & 15;
+        const int gr = 16 * i + row, gc = 16 * i + col;
+        const double v = G[min(gr, n - 1) + (size_t)min(gc, n - 1) * ldg];
+        double x = (gc < n) ? v : (gr == gc ? 1.0 : 0.0);
+        x = (row > col) ? 0.0 : ((gr == gc && gc < n) ? x + sh : x);
+        sG[i][col][row] = x;
+    }
+    // Off-diagonal blocks to registers in groups of four slots: the group's
+    // 16 loads are issued together (clamped addresses), then padded.
     dh_d4 a[kCholSlots];
 #pragma unroll
     for (int s0 = 0; s0 < kCholSlots; s0 += 4) {
@@ -565,34 +579,24 @@ __device__ __forceinline__ void chol_reg_body(int n, double* __restrict__ G, int
             const int s = s0 + j;
             if (s < kCholSlots) {
                 const int c = 16 * bJ[s] + li;
-                const bool dblk = bI[s] == bJ[s];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int row = 16 * bI[s] + g + 4 * r;
-                    double e = (row < n && c < n) ? v[j][r] : (row == c ? 1.0 : 0.0);
-                    e = (dblk && row > c) ? 0.0 : e;
-                    e = (dblk && row == c && c < n) ? e + sh : e;
-                    a[s][r] = e;
+                    a[s][r] = (row < n && c < n) ? v[j][r] : 0.0;
                 }
             }
         }
         __builtin_amdgcn_sched_barrier(0);  // keep the groups apart (register pressure)
     }
 
-    // Step A for panel k, by the wave holding block (k, k) in slot sd.
-    auto factor_diag = [&](int k, int sd) {
-#pragma unroll
-        for (int s = 0; s < kCholSlots; ++s)
-            if (s == sd)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) sD[li][g + 4 * r] = a[s][r];
-        wave_lds_sync();
+    // Step A for panel k (wave k % 8).
+    auto factor_diag = [&](int k) {
         // Lane j holds column j of the block (lanes 16..63 a copy of lane
         // j & 15, never read).  Entries below the diagonal are not needed and
         // take whatever the unmasked updates leave there.
         double u[NB];
 #pragma unroll
-        for (int i = 0; i < NB; ++i) u[i] = sD[li][i];
+        for (int i = 0; i < NB; ++i) u[i] = sG[k][li][i];
         const int nreal = n - 16 * k;
         int bad = 0;
 #pragma unroll
@@ -609,30 +613,44 @@ __device__ __forceinline__ void chol_reg_body(int n, double* __restrict__ G, int
                     y = fma(y, fma(-hp * y, y, 0.5), y);
                     const double ukj = (lane == kk) ? piv * y : u[kk] * y;
                     u[kk] = ukj;
+                    if (kk + 1 < NB) {
+                        double* row = sR[kk & 1];
+                        if (lane < NB) row[lane] = ukj;
+                        wave_lds_sync();
 #pragma unroll
-                    for (int i = kk + 1; i < NB; ++i) u[i] -= readlane_d(ukj, i) * ukj;
+                        for (int i = kk + 1; i < NB; ++i) u[i] -= row[i] * ukj;
+                    }
                 }
             }
         }
         const int c = 16 * k + lane;
-        if (lane < NB && c < n)
+        if (lane < NB) {
+#pragma unroll
+            for (int i = 0; i < NB; ++i) sU[i][lane] = u[i];
+            double dg = 1.0;
 #pragma unroll
             for (int i = 0; i < NB; ++i)
-                if (i <= lane) G[(16 * k + i) + (size_t)c * ldg] = u[i];
-        // Column j of U_kk^{-1} by back substitution, in place: step i reads
-        // row i of U (u[i] of lanes q > i) and then overwrites u[i] with x[i].
-        // Lane i holds 1 / U(i, i), formed once.
-        double dg = 1.0;
+                if (i == lane) dg = u[i];
+            sRd[lane] = 1.0 / dg;
+            if (c < n)
 #pragma unroll
-        for (int i = 0; i < NB; ++i)
-            if (i == lane) dg = u[i];
-        const double rdg = 1.0 / dg;
+                for (int i = 0; i < NB; ++i)
+                    if (i <= lane) G[(16 * k + i) + (size_t)c * ldg] = u[i];
+        }
+        wave_lds_sync();
+        // Column j of U_kk^{-1} by back substitution, in place: x[i] =
+        // (e_j[i] - sum_{q>i} U(i, q) x[q]) / U(i, i), two partial sums.
 #pragma unroll
         for (int i = NB - 1; i >= 0; --i) {
-            double t = (i == lane) ? 1.0 : 0.0;
+            double t0 = (i == lane) ? 1.0 : 0.0, t1 = 0.0;
 #pragma unroll
-            for (int q = i + 1; q < NB; ++q) t -= readlane_d(u[i], q) * u[q];
-            u[i] = t * readlane_d(rdg, i);
+            for (int q = i + 1; q < NB; ++q) {
+                if ((q - i) & 1)
+                    t0 -= sU[i][q] * u[q];
+                else
+                    t1 -= sU[i][q] * u[q];
+            }
+            u[i] = (t0 + t1) * sRd[i];
         }
         if (lane < NB)
 #pragma unroll
@@ -642,16 +660,30 @@ __device__ __forceinline__ void chol_reg_body(int n, double* __restrict__ G, int
             }
         if (lane == 0 && bad) s_info = 16 * k + bad;
     };
+    // Diagonal block i -= U_ki^T U_ki (panel k in sP), in LDS.
+    auto update_diag = [&](int i, int k) {
+        const int pi = 16 * (i - k - 1) + li;
+        dh_d4 d;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) d[r] = sG[i][li][g + 4 * r];
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+            const double v = sP[4 * s4 + g][pi];
+            d = Mfma<double>::op(-v, v, d);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sG[i][li][g + 4 * r] = d[r];
+    };
 
-    __syncthreads();  // s_info initialised
-    if (wave == 0) factor_diag(0, 0);  // block (0, 0) is block 0: wave 0, slot 0
+    __syncthreads();  // diagonal blocks and s_info in LDS
+    if (wave == 0) factor_diag(0);
     for (int k = 0;; ++k) {
         __syncthreads();  // U_kk, its inverse and s_info ready
         if (s_info || k + 1 == nbk) break;
         // ---- B: row panel U_kj = U_kk^{-T} A_kj.
 #pragma unroll
         for (int s = 0; s < kCholSlots; ++s) {
-            if (bI[s] == k && bJ[s] > k) {
+            if (bI[s] == k) {
                 dh_d4 acc = {0, 0, 0, 0};
 #pragma unroll
                 for (int s4 = 0; s4 < 4; ++s4)
@@ -669,25 +701,25 @@ __device__ __forceinline__ void chol_reg_body(int n, double* __restrict__ G, int
                 G[(16 * k + r) + (size_t)(16 * (k + 1) + cl) * ldg] = sP[r][cl];
             }
         }
-        // ---- C: trailing update A_ij -= U_ki^T U_kj, block (k+1, k+1)
-        // and step A of panel k+1 first on the wave that holds it.
-        const int kd = (k + 1) * nbk - (k + 1) * k / 2;  // number of block (k+1, k+1)
-        const int sd = (wave == kd % kCholWaves) ? kd / kCholWaves : -1;
-        auto update = [&](int s) {
-            const int pi = 16 * (bI[s] - k - 1) + li, pj = 16 * (bJ[s] - k - 1) + li;
-#pragma unroll
-            for (int s4 = 0; s4 < 4; ++s4)
-                a[s] = Mfma<double>::op(-sP[4 * s4 + g][pi], sP[4 * s4 + g][pj], a[s]);
-        };
-        if (sd >= 0) {
-#pragma unroll
-            for (int s = 0; s < kCholSlots; ++s)
-                if (s == sd) update(s);
-            factor_diag(k + 1, sd);
+        // ---- C: trailing update; block (k+1, k+1) and step A of panel k+1
+        // first on wave (k+1) % 8.
+        if (wave == (k + 1) % kCholWaves) {
+            update_diag(k + 1, k);
+            wave_lds_sync();
+            factor_diag(k + 1);
         }
+        for (int i = k + 2 + (wave - k - 2 + 8 * kCholWaves) % kCholWaves; i < nbk;
+             i += kCholWaves)
+            update_diag(i, k);
 #pragma unroll
-        for (int s = 0; s < kCholSlots; ++s)
-            if (bI[s] > k && bI[s] < nbk && s != sd) update(s);
+        for (int s = 0; s < kCholSlots; ++s) {
+            if (bI[s] > k && bI[s] < nbk) {
+                const int pi = 16 * (bI[s] - k - 1) + li, pj = 16 * (bJ[s] - k - 1) + li;
+#pragma unroll
+                for (int s4 = 0; s4 < 4; ++s4)
+                    a[s] = Mfma<double>::op(-sP[4 * s4 + g][pi], sP[4 * s4 + g][pj], a[s]);
+            }
+        }
     }
     if (tid == 0) *info = s_info;
 }
