@@ -534,11 +534,9 @@ __device__ __forceinline__ void chol_reg_body(int n, double* __restrict__ G, int
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int g = lane >> 4, li = lane & 15;
 
-    double tol, sh;
-    chol_scales(n, G, ldg, m_rows, shift, rel_check, tol, sh);
-    if (tid == 0) s_info = 0;
+    static_assert(NT >= 16 * kRegBlocks, "one thread per diagonal element");
+    double tol = 0.0;  // breakdown tolerance, set once the diagonal is in LDS
     const int nbk = (n + 15) / 16;
-
     int bI[kCholSlots], bJ[kCholSlots];  // block of each slot; bI == nbk: unused
 #pragma unroll
     for (int s = 0; s < kCholSlots; ++s) {
@@ -550,15 +548,28 @@ __device__ __forceinline__ void chol_reg_body(int n, double* __restrict__ G, int
         bI[s] = bi;
         bJ[s] = bi + 1 + rem;
     }
-    // Diagonal blocks to LDS: upper triangle, zero below, identity padding,
-    // shifted diagonal.
-    for (int e = tid; e < 256 * nbk; e += NT) {
-        const int i = e >> 8, row = e & 15, col = (e >> 4) & 15;
-        const int gr = 16 * i + row, gc = 16 * i + col;
-        const double v = G[min(gr, n - 1) + (size_t)min(gc, n - 1) * ldg];
-        double x = (gc < n) ? v : (gr == gc ? 1.0 : 0.0);
-        x = (row > col) ? 0.0 : ((gr == gc && gc < n) ? x + sh : x);
-        sG[i][col][row] = x;
+    // Diagonal blocks to LDS: upper triangle, zero below, identity padding
+    // (all loads issued first, clamped addresses).
+    {
+        constexpr int kIt = (256 * kRegBlocks + NT - 1) / NT;
+        const int ne = 256 * nbk;
+        double dv[kIt];
+#pragma unroll
+        for (int it = 0; it < kIt; ++it) {
+            const int e = min(tid + it * NT, ne - 1);
+            const int i = e >> 8, row = e & 15, col = (e >> 4) & 15;
+            dv[it] = G[min(16 * i + row, n - 1) + (size_t)min(16 * i + col, n - 1) * ldg];
+        }
+#pragma unroll
+        for (int it = 0; it < kIt; ++it) {
+            const int e = tid + it * NT;
+            if (e < ne) {
+                const int i = e >> 8, row = e & 15, col = (e >> 4) & 15;
+                const int gr = 16 * i + row, gc = 16 * i + col;
+                double x = (gc < n) ? dv[it] : (gr == gc ? 1.0 : 0.0);
+                sG[i][col][row] = (row > col) ? 0.0 : x;
+            }
+        }
     }
     // Off-diagonal blocks to registers in groups of four slots: the group's
     // 16 loads are issued together (clamped addresses), then padded.
@@ -675,7 +686,40 @@ __device__ __forceinline__ void chol_reg_body(int n, double* __restrict__ G, int
         for (int r = 0; r < 4; ++r) sG[i][li][g + 4 * r] = d[r];
     };
 
-    __syncthreads();  // diagonal blocks and s_info in LDS
+    __syncthreads();  // diagonal blocks in LDS
+    {
+        // Breakdown tolerance and CholeskyQR3 shift from the diagonal, as
+        // chol_scales computes them, then the shift applied in LDS.
+        __shared__ double red[2][kCholWaves];
+        double mx = 0.0, tr = 0.0;
+        if (tid < n) {
+            mx = sG[tid >> 4][tid & 15][tid & 15];
+            tr = mx;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            mx = fmax(mx, __shfl_xor(mx, o));
+            tr += __shfl_xor(tr, o);
+        }
+        if (lane == 0) {
+            red[0][wave] = mx;
+            red[1][wave] = tr;
+        }
+        __syncthreads();
+        double m2 = 0.0, t2 = 0.0;
+#pragma unroll
+        for (int w = 0; w < kCholWaves; ++w) {
+            m2 = fmax(m2, red[0][w]);
+            t2 += red[1][w];
+        }
+        tol = rel_check ? (double)n * 2.220446049250313e-16 * m2 : 0.0;
+        const double sh = shift ? 11.0 * ((double)m_rows * n + (double)n * (n + 1)) *
+                                      1.1102230246251565e-16 * t2
+                                : 0.0;
+        if (shift && tid < n) sG[tid >> 4][tid & 15][tid & 15] += sh;
+        if (tid == 0) s_info = 0;
+    }
+    __syncthreads();  // shifted diagonal blocks and s_info in LDS
     if (wave == 0) factor_diag(0);
     for (int k = 0;; ++k) {
         __syncthreads();  // U_kk, its inverse and s_info ready
