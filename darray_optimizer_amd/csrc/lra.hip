@@ -113,7 +113,7 @@ int dh_chol_small_max_order();
 int dh_chol_small_launch(dh_handle_t h, int n, double* G, int ldg, double* Dinv, int m_rows,
                          bool shift, bool rel_check, int* info_dev);
 int dh_trsm_rows_launch(dh_handle_t h, int m, int n, double* Y, int ldy, const double* U,
-                        int ldu, const double* Dinv);
+                        int ldu, const double* Dinv, const int* guard, int nguard);
 
 // One CholeskyQR sweep in place for orders <= 544: G = upper(Y^T Y) (SYRK),
 // one-workgroup Cholesky with the relative breakdown test, Y := Y U^{-1} by
@@ -128,7 +128,18 @@ static int cholqr_sweep_small(dh_handle_t h, int m, int n, double* Y, int ldy, d
     DH_CHECK_HIP(hipMemcpyAsync(info, dinfo, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     DH_CHECK_HIP(hipStreamSynchronize(h->stream));
     if (*info) return 0;
-    return dh_trsm_rows_launch(h, m, n, Y, ldy, G, n, Dinv);
+    return dh_trsm_rows_launch(h, m, n, Y, ldy, G, n, Dinv, nullptr, 0);
+}
+
+// The same sweep with no host synchronisation: the Cholesky status goes to
+// dinfo[slot] and the solve is skipped on the device unless dinfo[0 .. slot]
+// are all zero, so a failed sweep leaves Y untouched.
+static int cholqr_sweep_small_async(dh_handle_t h, int m, int n, double* Y, int ldy, double* G,
+                                    double* Dinv, int* dinfo, int slot) {
+    DH_CHECK_STATUS(
+        dh_gemm_impl<double>(h, true, false, n, n, m, 1.0, Y, ldy, Y, ldy, 0.0, G, n, 1));
+    DH_CHECK_STATUS(dh_chol_small_launch(h, n, G, n, Dinv, m, false, true, dinfo + slot));
+    return dh_trsm_rows_launch(h, m, n, Y, ldy, G, n, Dinv, dinfo, slot + 1);
 }
 
 // R accumulation across sweeps: Racc = U_sweep * Racc (both upper).
@@ -195,15 +206,31 @@ int dh_cholqr2_impl(dh_handle_t h, int m, int n, double* Y, int ldy, double* R, 
     double *Racc = G + nn, *Rtmp = G + 2 * nn;  // Rtmp spans 2 nn
     double* Dbuf = (double*)dh_ws(h, DH_WS_RBUF, sizeof(double) * 256 * (size_t)dh_cdiv(n, 16));
     if (!Dbuf) return DH_ERR_ALLOC;
-    int info = 0, sweeps = 2;
-    DH_CHECK_STATUS(cholqr_sweep_small(h, m, n, Y, ldy, G, Dbuf, false, &info));
-    if (info) {
-        // A failed sweep leaves Y untouched: restart with the shifted Gram.
+    // Both sweeps queued back to back, one host synchronisation for the two
+    // statuses.
+    int* dinfo = (int*)dh_ws(h, DH_WS_INFO, 256);
+    if (!dinfo) return DH_ERR_ALLOC;
+    DH_CHECK_STATUS(cholqr_sweep_small_async(h, m, n, Y, ldy, G, Dbuf, dinfo, 0));
+    if (R) DH_CHECK_STATUS(accumulate_r(h, n, G, Racc, Rtmp, true, nullptr, ldr));
+    DH_CHECK_STATUS(cholqr_sweep_small_async(h, m, n, Y, ldy, G, Dbuf, dinfo, 1));
+    if (R) DH_CHECK_STATUS(accumulate_r(h, n, G, Racc, Rtmp, false, R, ldr));
+    int hinfo[2] = {0, 0};
+    DH_CHECK_HIP(hipMemcpyAsync(hinfo, dinfo, sizeof(hinfo), hipMemcpyDeviceToHost, h->stream));
+    DH_CHECK_HIP(hipStreamSynchronize(h->stream));
+    if (hinfo[0] == 0) {
+        if (hinfo[1] == 0) return 0;
+        return dh_fail(1000000 + hinfo[1],
+                       "CholeskyQR: Gram matrix of sweep 2 is numerically singular (pivot %d)",
+                       hinfo[1]);
+    }
+    int info = 0, sweeps = 3;
+    {
+        // The first sweep broke down and both solves were skipped, so Y is
+        // untouched: restart with the shifted Gram (shifted CholeskyQR3).
         DH_CHECK_STATUS(cholqr_sweep_small(h, m, n, Y, ldy, G, Dbuf, true, &info));
         if (info)
             return dh_fail(info, "CholeskyQR: shifted Gram matrix not positive definite (pivot %d)",
                            info);
-        sweeps = 3;
     }
     if (R) DH_CHECK_STATUS(accumulate_r(h, n, G, Racc, Rtmp, true, nullptr, ldr));
     for (int sw = 2; sw <= sweeps; ++sw) {

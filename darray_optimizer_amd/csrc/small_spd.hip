@@ -454,9 +454,13 @@ __global__ __launch_bounds__(WAVES * 64) void trsm_rows_reg_kernel(int m, int n,
                                                                    int ldy,
                                                                    const double* __restrict__ U,
                                                                    int ldu,
-                                                                   const double* __restrict__ Dinv) {
+                                                                   const double* __restrict__ Dinv,
+                                                                   const int* __restrict__ guard,
+                                                                   int nguard) {
     __shared__ double tile[WAVES][16][17];
     __shared__ double sU[16 * (kRegBlocks - 1)][17];
+    // Whole-workgroup exit when an earlier Cholesky broke down.
+    if (guard && (guard[0] | (nguard > 1 ? guard[1] : 0))) return;
     const int wave = threadIdx.x >> 6;
     const int r0 = (blockIdx.x * WAVES + wave) * 16;
     trsm_rows_reg_body<kRegBlocks, WAVES>(r0, m, n, Y, ldy, U, ldu, Dinv, tile[wave], sU);
@@ -482,7 +486,10 @@ __global__ __launch_bounds__(WAVES * 64) void trsm_rows_kernel(int m, int n,
                                                                const double* __restrict__ U,
                                                                int ldu,
                                                                const double* __restrict__ Dinv,
-                                                               int pitch) {
+                                                               int pitch,
+                                                               const int* __restrict__ guard,
+                                                               int nguard) {
+    if (guard && (guard[0] | (nguard > 1 ? guard[1] : 0))) return;  // a Cholesky broke down
     trsm_rows_body<WAVES>(blockIdx.x, m, n, Y, ldy, U, ldu, Dinv, pitch);
 }
 
@@ -836,14 +843,17 @@ int dh_cholqr_step_batched(dh_handle_t h, const DhSpdJob* jobs, int count, int m
 }
 
 // Y := Y U^{-1} with U from chol_small (upper, order n <= 544) and its Dinv.
+// With guard != nullptr the kernel does nothing unless guard[0 .. nguard-1]
+// (device status words of the Cholesky steps before it, nguard <= 2) are
+// all zero.
 int dh_trsm_rows_launch(dh_handle_t h, int m, int n, double* Y, int ldy, const double* U,
-                        int ldu, const double* Dinv) {
+                        int ldu, const double* Dinv, const int* guard, int nguard) {
     if (n > kMaxN) return dh_fail(DH_ERR_UNSUPPORTED, "trsm_rows: order %d > %d", n, kMaxN);
     if (m == 0 || n == 0) return 0;
     if (n <= 16 * kRegBlocks) {
         constexpr int WR = 4;
         trsm_rows_reg_kernel<WR><<<dh_cdiv(m, 16 * WR), WR * 64, 0, h->stream>>>(m, n, Y, ldy, U,
-                                                                               ldu, Dinv);
+                                                                               ldu, Dinv, guard, nguard);
         DH_CHECK_LAUNCH();
         return 0;
     }
@@ -858,7 +868,7 @@ int dh_trsm_rows_launch(dh_handle_t h, int m, int n, double* Y, int ldy, const d
         lds_attr_set = true;
     }
     trsm_rows_kernel<W><<<dh_cdiv(m, 16 * W), W * 64, lds, h->stream>>>(m, n, Y, ldy, U, ldu,
-                                                                          Dinv, pitch);
+                                                                          Dinv, pitch, guard, nguard);
     DH_CHECK_LAUNCH();
     return 0;
 }
