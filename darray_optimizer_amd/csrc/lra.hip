@@ -110,6 +110,16 @@ static int cholqr_sweep(dh_handle_t h, int m, int n, double* Y, int ldy, double*
 }
 
 int dh_chol_small_max_order();
+int dh_gram_small_launch(dh_handle_t h, int m, int n, const double* Y, int ldy, double* G,
+                         int ldg);
+
+// G = upper(Y^T Y): the register-blocked Gram kernel where it applies
+// (128 < n <= 272), else the tiled GEMM.
+static int cholqr_gram(dh_handle_t h, int m, int n, const double* Y, int ldy, double* G) {
+    const int st = dh_gram_small_launch(h, m, n, Y, ldy, G, n);
+    if (st != 1) return st;
+    return dh_gemm_impl<double>(h, true, false, n, n, m, 1.0, Y, ldy, Y, ldy, 0.0, G, n, 1);
+}
 int dh_chol_small_launch(dh_handle_t h, int n, double* G, int ldg, double* Dinv, int m_rows,
                          bool shift, bool rel_check, int* info_dev);
 int dh_trsm_rows_launch(dh_handle_t h, int m, int n, double* Y, int ldy, const double* U,
@@ -122,8 +132,7 @@ static int cholqr_sweep_small(dh_handle_t h, int m, int n, double* Y, int ldy, d
                               double* Dinv, bool shift, int* info) {
     int* dinfo = (int*)dh_ws(h, DH_WS_INFO, 256);
     if (!dinfo) return DH_ERR_ALLOC;
-    DH_CHECK_STATUS(
-        dh_gemm_impl<double>(h, true, false, n, n, m, 1.0, Y, ldy, Y, ldy, 0.0, G, n, 1));
+    DH_CHECK_STATUS(cholqr_gram(h, m, n, Y, ldy, G));
     DH_CHECK_STATUS(dh_chol_small_launch(h, n, G, n, Dinv, m, shift, true, dinfo));
     DH_CHECK_HIP(hipMemcpyAsync(info, dinfo, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     DH_CHECK_HIP(hipStreamSynchronize(h->stream));
@@ -136,8 +145,7 @@ static int cholqr_sweep_small(dh_handle_t h, int m, int n, double* Y, int ldy, d
 // are all zero, so a failed sweep leaves Y untouched.
 static int cholqr_sweep_small_async(dh_handle_t h, int m, int n, double* Y, int ldy, double* G,
                                     double* Dinv, int* dinfo, int slot) {
-    DH_CHECK_STATUS(
-        dh_gemm_impl<double>(h, true, false, n, n, m, 1.0, Y, ldy, Y, ldy, 0.0, G, n, 1));
+    DH_CHECK_STATUS(cholqr_gram(h, m, n, Y, ldy, G));
     DH_CHECK_STATUS(dh_chol_small_launch(h, n, G, n, Dinv, m, false, true, dinfo + slot));
     return dh_trsm_rows_launch(h, m, n, Y, ldy, G, n, Dinv, dinfo, slot + 1);
 }

@@ -1,5 +1,6 @@
-// The k x k step of CholeskyQR on gfx950, and the row-parallel triangular
-// solve that applies its factor to the n x k panel.
+// The k x k step of CholeskyQR on gfx950, the row-parallel triangular solve
+// that applies its factor to the n x k panel, and (128 < k <= 272) the Gram
+// product itself with the whole upper k x k result held in registers.
 //
 // chol_reg_kernel (one workgroup, order n <= 272: the matrix held in
 // registers) and chol_small_kernel (order n <= 544): G (upper triangle read)
@@ -789,9 +790,172 @@ __global__ __launch_bounds__(NT) void chol_reg_batched_kernel(const DhSpdJob* __
     if (threadIdx.x == 0 && *j.info != 0) *j.sticky = 1;
 }
 
+// Gram G = upper(Y^T Y) of an m x n panel, 128 < n <= 16 * kRegBlocks: the
+// CholeskyQR product, whose 64 x 64-tiled GEMM would pad 272 to 320.  A
+// workgroup (8 waves) takes a slab of rows and accumulates all upper 16 x 16
+// blocks of the order-16*kRegBlocks Gram in registers (FP64 MFMA accumulator
+// layout); block b (numbered row by row, 153 of them) belongs to wave b % 8,
+// fixed at compile time so every operand index is static.  The slab streams
+// through LDS in 16-row chunks (double buffered, the next chunk's loads in
+// flight during the MFMAs); per 4-row step a wave reads the 17 column
+// operands once and issues its blocks' MFMAs (A = B layout: lane (g, li)
+// holds Y(k + g, 16 c + li) for column block c).  Partial sums go to
+// part[z][block][r][lane] and gram_reduce_kernel adds the slabs.
+constexpr int kGramBlocks = kRegBlocks * (kRegBlocks + 1) / 2;           // 153
+constexpr int kGramSlots = (kGramBlocks + kCholWaves - 1) / kCholWaves;  // 20
+constexpr int kGramPitch = 16 * kRegBlocks + 4;
+
+struct GramMap {
+    int bi[kGramBlocks], bj[kGramBlocks];
+};
+__host__ __device__ constexpr GramMap make_gram_map() {
+    GramMap m{};
+    int b = 0;
+    for (int i = 0; i < kRegBlocks; ++i)
+        for (int j = i; j < kRegBlocks; ++j) {
+            m.bi[b] = i;
+            m.bj[b] = j;
+            ++b;
+        }
+    return m;
+}
+constexpr GramMap kGramMap = make_gram_map();
+
+// acc[S] += Y_bi^T Y_bj for the blocks of wave W (indices fixed at compile time).
+template <int W, int S>
+__device__ __forceinline__ void gram_mfma(dh_d4 (&acc)[kGramSlots],
+                                          const double (&op)[kRegBlocks], int nbk) {
+    if constexpr (S < kGramSlots) {
+        constexpr int b = S * kCholWaves + W;
+        if constexpr (b < kGramBlocks) {
+            constexpr int bi = kGramMap.bi[b], bj = kGramMap.bj[b];
+            if (bj < nbk) acc[S] = Mfma<double>::op(op[bi], op[bj], acc[S]);
+        }
+        gram_mfma<W, S + 1>(acc, op, nbk);
+    }
+}
+
+template <int W>
+__device__ __forceinline__ void gram_body(int m, int n, const double* __restrict__ Y, int ldy,
+                                          int r0, int r1, double* __restrict__ part,
+                                          double (*sY)[16][kGramPitch]) {
+    const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, li = lane & 15;
+    const int nbk = (n + 15) / 16;
+    constexpr int kIt = (16 * 16 * kRegBlocks + NT - 1) / NT;
+    dh_d4 acc[kGramSlots];
+#pragma unroll
+    for (int s = 0; s < kGramSlots; ++s) acc[s] = dh_d4{0, 0, 0, 0};
+    // Chunk element e: row e & 15, column e >> 4 (16-row runs of a column).
+    auto load = [&](int c0, double (&v)[kIt]) {
+#pragma unroll
+        for (int it = 0; it < kIt; ++it) {
+            const int e = tid + it * NT, row = c0 + (e & 15), col = min(e >> 4, n - 1);
+            v[it] = Y[min(row, m - 1) + (size_t)col * ldy];
+        }
+    };
+    auto store = [&](int c0, int buf, const double (&v)[kIt]) {
+#pragma unroll
+        for (int it = 0; it < kIt; ++it) {
+            const int e = tid + it * NT, row = c0 + (e & 15), col = e >> 4;
+            if (col < 16 * kRegBlocks)
+                sY[buf][e & 15][col] = (row < r1 && col < n) ? v[it] : 0.0;
+        }
+    };
+    double v[kIt];
+    load(r0, v);
+    store(r0, 0, v);
+    __syncthreads();
+    int buf = 0;
+    for (int c0 = r0; c0 < r1; c0 += 16) {
+        const bool more = c0 + 16 < r1;
+        if (more) load(c0 + 16, v);
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+            double op[kRegBlocks];
+#pragma unroll
+            for (int c = 0; c < kRegBlocks; ++c) op[c] = sY[buf][4 * s4 + g][16 * c + li];
+            gram_mfma<W, 0>(acc, op, nbk);
+        }
+        if (more) store(c0 + 16, buf ^ 1, v);
+        __syncthreads();
+        buf ^= 1;
+    }
+    double* pz = part + (size_t)blockIdx.x * kGramBlocks * 256;
+#pragma unroll
+    for (int s = 0; s < kGramSlots; ++s) {
+        const int b = s * kCholWaves + W;
+        if (b < kGramBlocks)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) pz[(size_t)b * 256 + r * 64 + lane] = acc[s][r];
+    }
+}
+
+__global__ __launch_bounds__(NT) void gram_reg_kernel(int m, int n, const double* __restrict__ Y,
+                                                      int ldy, int rows, double* __restrict__ part) {
+    __shared__ double sY[2][16][kGramPitch];
+    const int r0 = blockIdx.x * rows, r1 = min(m, r0 + rows);
+    switch (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) {
+        case 0: gram_body<0>(m, n, Y, ldy, r0, r1, part, sY); break;
+        case 1: gram_body<1>(m, n, Y, ldy, r0, r1, part, sY); break;
+        case 2: gram_body<2>(m, n, Y, ldy, r0, r1, part, sY); break;
+        case 3: gram_body<3>(m, n, Y, ldy, r0, r1, part, sY); break;
+        case 4: gram_body<4>(m, n, Y, ldy, r0, r1, part, sY); break;
+        case 5: gram_body<5>(m, n, Y, ldy, r0, r1, part, sY); break;
+        case 6: gram_body<6>(m, n, Y, ldy, r0, r1, part, sY); break;
+        default: gram_body<7>(m, n, Y, ldy, r0, r1, part, sY); break;
+    }
+}
+
+// G(i, j), i <= j < n, = sum over the nz slabs of part: 64 elements per
+// workgroup, four thread groups over interleaved slabs, then an LDS sum.
+__global__ __launch_bounds__(256) void gram_reduce_kernel(int n, int nz,
+                                                          const double* __restrict__ part,
+                                                          double* __restrict__ G, int ldg) {
+    __shared__ double red[4][64];
+    const int e = blockIdx.x * 64 + (threadIdx.x & 63), zg = threadIdx.x >> 6;
+    double s0 = 0.0, s1 = 0.0;
+    if (e < kGramBlocks * 256) {
+        const size_t stride = (size_t)kGramBlocks * 256;
+        int z = zg;
+        for (; z + 4 < nz; z += 8) {
+            s0 += part[(size_t)z * stride + e];
+            s1 += part[(size_t)(z + 4) * stride + e];
+        }
+        if (z < nz) s0 += part[(size_t)z * stride + e];
+    }
+    red[zg][threadIdx.x & 63] = s0 + s1;
+    __syncthreads();
+    if (zg == 0 && e < kGramBlocks * 256) {
+        const double t = (red[0][threadIdx.x] + red[1][threadIdx.x]) +
+                         (red[2][threadIdx.x] + red[3][threadIdx.x]);
+        const int b = e >> 8, r = (e >> 6) & 3, ln = e & 63;
+        const int row = 16 * kGramMap.bi[b] + (ln >> 4) + 4 * r;
+        const int col = 16 * kGramMap.bj[b] + (ln & 15);
+        if (col < n && row <= col) G[row + (size_t)col * ldg] = t;
+    }
+}
+
 }  // namespace
 
 int dh_chol_small_max_order() { return kMaxN; }
+
+// G = upper(Y^T Y) for the CholeskyQR Gram when 128 < n <= 272 (returns 1,
+// nothing launched, for other orders: the caller uses the tiled GEMM).
+int dh_gram_small_launch(dh_handle_t h, int m, int n, const double* Y, int ldy, double* G,
+                         int ldg) {
+    if (n <= 128 || n > 16 * kRegBlocks || m < 16) return 1;
+    // One slab per CU: eight waves at 256 registers fill a CU's register file.
+    int nz = min(h->num_cu, dh_cdiv(m, 16));
+    const int rows = 16 * dh_cdiv(dh_cdiv(m, nz), 16);
+    nz = dh_cdiv(m, rows);
+    double* part = (double*)dh_ws(h, DH_WS_SPLITK, sizeof(double) * 256 * kGramBlocks * (size_t)nz);
+    if (!part) return DH_ERR_ALLOC;
+    gram_reg_kernel<<<nz, NT, 0, h->stream>>>(m, n, Y, ldy, rows, part);
+    DH_CHECK_LAUNCH();
+    gram_reduce_kernel<<<dh_cdiv(kGramBlocks * 256, 64), 256, 0, h->stream>>>(n, nz, part, G, ldg);
+    DH_CHECK_LAUNCH();
+    return 0;
+}
 
 // One-workgroup Cholesky of G (+ 16 x 16 diagonal-block inverses into Dinv,
 // which needs 256 * ceil(n/16) doubles); *info_dev receives the status.
