@@ -389,33 +389,57 @@ __device__ __forceinline__ void trsm_rows_reg_body(int r0, int m, int n, double*
                                                    int ldy, const double* __restrict__ U,
                                                    int ldu, const double* __restrict__ Dinv,
                                                    double (*tile)[17], double (*sU)[17]) {
+    constexpr int NT_ = WAVES * 64;
+    static_assert(NT_ == 256, "block column J of U is 256 J values, J per thread");
     const int lane = threadIdx.x & 63, g = lane >> 4, li = lane & 15;
     const int nblk = (n + 15) / 16;
+    const int rowA = r0 + li;  // A-operand layout: lane (g, li) holds row li, column 4s + g
     double qa[NBM][4];
+    // Loads of block column J, issued one block ahead: Y and Dinv in the
+    // A-operand layout (16-row runs of a column), U's rows above the block.
+    double yt[4], dv[4], pf[NBM > 1 ? NBM - 1 : 1];
+    auto load_y = [&](int J) {
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+            const int col = 16 * J + 4 * s4 + g;
+            const double v = Y[min(rowA, m - 1) + (size_t)min(col, n - 1) * ldy];
+            yt[s4] = col < n ? v : 0.0;
+            dv[s4] = Dinv[256 * J + (4 * s4 + g) + 16 * li];
+        }
+    };
+    load_y(0);
 #pragma unroll
     for (int J = 0; J < NBM; ++J) {
         if (J < nblk) {  // uniform over the workgroup
-            const int col = 16 * J + li;
-            const bool cok = col < n;
-            double yv[4], dv[4];
+            double yc[4], dc[4];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = r0 + g + 4 * r;
-                yv[r] = (row < m && cok) ? Y[row + (size_t)col * ldy] : 0.0;
+            for (int s4 = 0; s4 < 4; ++s4) {
+                yc[s4] = yt[s4];
+                dc[s4] = dv[s4];
             }
-#pragma unroll
-            for (int ks = 0; ks < 4; ++ks) dv[ks] = Dinv[256 * J + (4 * ks + g) + 16 * li];
             if (J > 0) {
-                // sU[k][c] = U[k][16J + c] for k < 16J (zero past column n).
+                // sU[k][c] = U[k][16J + c] for k < 16J (zero past column n),
+                // from the registers loaded during block J - 1.
                 __syncthreads();  // the previous block column's readers are done
-                constexpr int NT_ = WAVES * 64;
-                for (int e = threadIdx.x; e < 256 * J; e += NT_) {
+#pragma unroll
+                for (int i = 0; i < J; ++i) {
+                    const int e = threadIdx.x + i * NT_;
                     const int k = e % (16 * J), c = e / (16 * J);
-                    const bool ok = 16 * J + c < n;
-                    const double v = U[k + (size_t)min(16 * J + c, n - 1) * ldu];
-                    sU[k][c] = ok ? v : 0.0;
+                    sU[k][c] = (16 * J + c < n) ? pf[i] : 0.0;
                 }
                 __syncthreads();
+            }
+            if (J + 1 < nblk) {
+                load_y(J + 1);
+                const int J1 = J + 1;  // a constant once the J loop is unrolled
+                if (J1 < NBM) {
+#pragma unroll
+                    for (int i = 0; i < J1; ++i) {
+                        const int e = threadIdx.x + i * NT_;
+                        const int k = e % (16 * J1), c = e / (16 * J1);
+                        pf[i] = U[k + (size_t)min(16 * J1 + c, n - 1) * ldu];
+                    }
+                }
             }
             // acc = sum_{K<J} Q_K U_KJ, one accumulator chain per k-slot s.
             dh_d4 acc[4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
@@ -425,23 +449,24 @@ __device__ __forceinline__ void trsm_rows_reg_body(int r0, int m, int n, double*
                 for (int s4 = 0; s4 < 4; ++s4)
                     acc[s4] = Mfma<double>::op(qa[K][s4], sU[16 * K + 4 * s4 + g][li], acc[s4]);
             const dh_d4 a = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-            // T = Y_J - acc (row g + 4r, column li) -> operand layout -> Q_J = T U_JJ^{-1}.
+            // acc to the A-operand layout, T = Y_J - acc, Q_J = T U_JJ^{-1}.
 #pragma unroll
-            for (int r = 0; r < 4; ++r) tile[g + 4 * r][li] = yv[r] - a[r];
+            for (int r = 0; r < 4; ++r) tile[g + 4 * r][li] = a[r];
             wave_lds_sync();
             dh_d4 q = {0, 0, 0, 0};
 #pragma unroll
-            for (int ks = 0; ks < 4; ++ks) q = Mfma<double>::op(tile[li][4 * ks + g], dv[ks], q);
+            for (int ks = 0; ks < 4; ++ks)
+                q = Mfma<double>::op(yc[ks] - tile[li][4 * ks + g], dc[ks], q);
             wave_lds_sync();
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = r0 + g + 4 * r;
-                if (row < m && cok) Y[row + (size_t)col * ldy] = q[r];
-                tile[g + 4 * r][li] = q[r];
+            for (int r = 0; r < 4; ++r) tile[g + 4 * r][li] = q[r];
+            wave_lds_sync();
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                qa[J][s4] = tile[li][4 * s4 + g];
+                const int col = 16 * J + 4 * s4 + g;
+                if (rowA < m && col < n) Y[rowA + (size_t)col * ldy] = qa[J][s4];
             }
-            wave_lds_sync();
-#pragma unroll
-            for (int s4 = 0; s4 < 4; ++s4) qa[J][s4] = tile[li][4 * s4 + g];
             wave_lds_sync();
         }
     }
