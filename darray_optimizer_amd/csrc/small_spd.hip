@@ -530,6 +530,49 @@ __global__ __launch_bounds__(WAVES * 64) void trsm_rows_batched_kernel(
                           16 * ((j.n + 15) / 16) + 1);
 }
 
+// Lane I of each 16-lane row, broadcast to the row (DPP row_newbcast): no
+// LDS round trip and no scalar registers.
+template <int I>
+__device__ __forceinline__ double bcast16(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x150 + I, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x150 + I, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// u[I..15] -= U(KK, I) * ukj, U(KK, I) being ukj of lane I.
+template <int I>
+__device__ __forceinline__ void chol16_update(double (&u)[NB], double ukj) {
+    if constexpr (I < NB) {
+        u[I] -= bcast16<I>(ukj) * ukj;
+        chol16_update<I + 1>(u, ukj);
+    }
+}
+
+// Column steps KK..15 of the 16 x 16 Cholesky, lane j (mod 16) holding
+// column j; bad receives the 1-based failing step.
+template <int KK>
+__device__ __forceinline__ void chol16_steps(double (&u)[NB], int li, int nreal, double tol,
+                                             int& bad) {
+    if constexpr (KK < NB) {
+        if (!bad) {
+            const double piv = readlane_d(u[KK], KK);
+            if (KK < nreal && !(piv > tol)) {
+                bad = KK + 1;
+            } else {
+                // 1/sqrt(piv): hardware estimate and two Newton steps.
+                const double hp = 0.5 * piv;
+                double y = __builtin_amdgcn_rsq(piv);
+                y = fma(y, fma(-hp * y, y, 0.5), y);
+                y = fma(y, fma(-hp * y, y, 0.5), y);
+                const double ukj = (li == KK) ? piv * y : u[KK] * y;
+                u[KK] = ukj;
+                chol16_update<KK + 1>(u, ukj);
+            }
+        }
+        chol16_steps<KK + 1>(u, li, nreal, tol, bad);
+    }
+}
+
 // Register-resident Cholesky for orders n <= 16 * kRegBlocks (= 272), same
 // contract as chol_small_body.  The order-16*nb matrix (nb = ceil(n/16); rows
 // and columns past n hold the identity, so they factor to the identity and
@@ -539,7 +582,8 @@ __global__ __launch_bounds__(WAVES * 64) void trsm_rows_batched_kernel(
 // robin; the diagonal blocks in LDS (sG[i][col][row]), block i updated by
 // wave i % 8.  Per 16-column panel k:
 //   A  wave k % 8 factors block (k, k) (column per lane; rows of U broadcast
-//      through LDS), inverts the factor by back substitution, and writes
+//      by DPP within 16-lane rows), inverts the factor by back substitution
+//      (rows of U from LDS), and writes
 //      U_kk to G, its inverse to Dinv and to LDS;
 //   B  the owners of blocks (k, j > k) form U_kj = U_kk^{-T} A_kj (4 MFMAs,
 //      A_kj's accumulator rows are the B operand) into the LDS panel, which
@@ -560,7 +604,6 @@ __device__ __forceinline__ void chol_reg_body(int n, double* __restrict__ G, int
     __shared__ double sI[NB][NB + 1];  // inverse factor of panel k, sI[j][i] = U_kk^{-1}(i, j)
     __shared__ __align__(16) double sP[NB][16 * (kRegBlocks - 1) + 4];  // U_k,k+1..nb-1
     __shared__ __align__(16) double sU[NB][NB];  // U_kk^T for the back substitution
-    __shared__ __align__(16) double sR[2][NB];   // row kk of U_kk
     __shared__ double sRd[NB];                   // 1 / U_kk(i, i)
     __shared__ int s_info;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -635,38 +678,15 @@ __device__ __forceinline__ void chol_reg_body(int n, double* __restrict__ G, int
 
     // Step A for panel k (wave k % 8).
     auto factor_diag = [&](int k) {
-        // Lane j holds column j of the block (lanes 16..63 a copy of lane
-        // j & 15, never read).  Entries below the diagonal are not needed and
-        // take whatever the unmasked updates leave there.
+        // Lane j holds column j & 15 of the block (every 16-lane row a full
+        // copy, so row broadcasts serve all lanes).  Entries below the
+        // diagonal are not needed and take whatever the unmasked updates
+        // leave there.
         double u[NB];
 #pragma unroll
         for (int i = 0; i < NB; ++i) u[i] = sG[k][li][i];
-        const int nreal = n - 16 * k;
         int bad = 0;
-#pragma unroll
-        for (int kk = 0; kk < NB; ++kk) {
-            if (!bad) {
-                const double piv = readlane_d(u[kk], kk);
-                if (kk < nreal && !(piv > tol)) {
-                    bad = kk + 1;
-                } else {
-                    // 1/sqrt(piv): hardware estimate and two Newton steps.
-                    const double hp = 0.5 * piv;
-                    double y = __builtin_amdgcn_rsq(piv);
-                    y = fma(y, fma(-hp * y, y, 0.5), y);
-                    y = fma(y, fma(-hp * y, y, 0.5), y);
-                    const double ukj = (lane == kk) ? piv * y : u[kk] * y;
-                    u[kk] = ukj;
-                    if (kk + 1 < NB) {
-                        double* row = sR[kk & 1];
-                        if (lane < NB) row[lane] = ukj;
-                        wave_lds_sync();
-#pragma unroll
-                        for (int i = kk + 1; i < NB; ++i) u[i] -= row[i] * ukj;
-                    }
-                }
-            }
-        }
+        chol16_steps<0>(u, li, n - 16 * k, tol, bad);
         const int c = 16 * k + lane;
         if (lane < NB) {
 #pragma unroll
