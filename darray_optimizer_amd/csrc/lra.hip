@@ -6,6 +6,8 @@
 // again for K*Q.  Here the orthogonalisation is CholeskyQR2 (Gram on MFMA,
 // small blocked Cholesky, right triangular solve, twice) and the K-passes
 // never materialise K.
+#include <utility>
+
 #include "dh_common.h"
 
 int dh_project_f64_impl(dh_handle_t h, int m, int n, int d, int l, const double* Xi, int ldxi,
@@ -309,7 +311,7 @@ static int copy_panel(dh_handle_t h, int n, int l, const double* src, int lds, d
     return 0;
 }
 
-// FP64 passes: Y = K*Omega -> Q; q x { K*Q -> Q }; W = K*Q.
+// FP64 passes: Y = K*Omega -> Q; q x { K*Q -> orthonormal basis }; W = K*Q.
 static int lra_f64(dh_handle_t h, int n, int d, int l, int q, const double* X, int ldx,
                    const double* y, double gamma, const double* Omega, int ldo, double* Q,
                    int ldq, double* W, int ldw) {
@@ -319,12 +321,17 @@ static int lra_f64(dh_handle_t h, int n, int d, int l, int q, const double* X, i
     DH_CHECK_STATUS(dh_project_f64_impl(h, n, n, d, l, X, ldx, X, ldx, nrm, nrm, y, y, gamma,
                                         Omega, ldo, 0.0, Q, ldq));
     DH_CHECK_STATUS(dh_range_orth(h, n, l, Q, ldq));
+    // Power iterations alternate the basis between Q and W (no copy for even q).
+    double *cur = Q, *nxt = W;
+    int ldc = ldq, ldn = ldw;
     for (int it = 0; it < q; ++it) {
         DH_CHECK_STATUS(dh_project_f64_impl(h, n, n, d, l, X, ldx, X, ldx, nrm, nrm, y, y, gamma,
-                                            Q, ldq, 0.0, W, ldw));
-        DH_CHECK_STATUS(dh_range_orth(h, n, l, W, ldw));
-        DH_CHECK_STATUS(copy_panel(h, n, l, W, ldw, Q, ldq));
+                                            cur, ldc, 0.0, nxt, ldn));
+        DH_CHECK_STATUS(dh_range_orth(h, n, l, nxt, ldn));
+        std::swap(cur, nxt);
+        std::swap(ldc, ldn);
     }
+    if (cur != Q) DH_CHECK_STATUS(copy_panel(h, n, l, cur, ldc, Q, ldq));
     return dh_project_f64_impl(h, n, n, d, l, X, ldx, X, ldx, nrm, nrm, y, y, gamma, Q, ldq, 0.0,
                                W, ldw);
 }

@@ -5,6 +5,8 @@
 // The fused projection kernels themselves are in rbf16_kernels.h.
 // Reference: DArray::LRA (dkernel.h:12-46) over rbf_kernel + tc_gemm
 // (dkernel.cu:132-231).
+#include <utility>
+
 #include "rbf16_kernels.h"
 
 namespace {
@@ -508,14 +510,20 @@ int lra_16_passes(dh_handle_t h, int n, int d, int l, int q, const double* X, in
     DH_CHECK_STATUS(pack_omega<F16>(h, n, l, Omega, ldo, y, Op));
     DH_CHECK_STATUS(project_16<F16>(h, n, n, d, l, p, p, c2, Op, y, Q, ldq));
     DH_CHECK_STATUS(dh_range_orth(h, n, l, Q, ldq));
+    // Power iterations alternate the basis between Q and W (no copy for even q).
+    double *cur = Q, *nxt = W;
+    int ldc = ldq, ldn = ldw;
     for (int it = 0; it < q; ++it) {
-        DH_CHECK_STATUS(pack_omega<F16>(h, n, l, Q, ldq, y, Op));
-        DH_CHECK_STATUS(project_16<F16>(h, n, n, d, l, p, p, c2, Op, y, W, ldw));
-        DH_CHECK_STATUS(dh_range_orth(h, n, l, W, ldw));
-        DH_CHECK_HIP(hipMemcpy2DAsync(Q, sizeof(double) * (size_t)ldq, W,
-                                      sizeof(double) * (size_t)ldw, sizeof(double) * (size_t)n, l,
-                                      hipMemcpyDeviceToDevice, h->stream));
+        DH_CHECK_STATUS(pack_omega<F16>(h, n, l, cur, ldc, y, Op));
+        DH_CHECK_STATUS(project_16<F16>(h, n, n, d, l, p, p, c2, Op, y, nxt, ldn));
+        DH_CHECK_STATUS(dh_range_orth(h, n, l, nxt, ldn));
+        std::swap(cur, nxt);
+        std::swap(ldc, ldn);
     }
+    if (cur != Q)
+        DH_CHECK_HIP(hipMemcpy2DAsync(Q, sizeof(double) * (size_t)ldq, cur,
+                                      sizeof(double) * (size_t)ldc, sizeof(double) * (size_t)n, l,
+                                      hipMemcpyDeviceToDevice, h->stream));
     DH_CHECK_STATUS(pack_omega<F16>(h, n, l, Q, ldq, y, Op));
     return project_16<F16>(h, n, n, d, l, p, p, c2, Op, y, W, ldw);
 }
@@ -539,14 +547,20 @@ int dh_lra_fp8_passes(dh_handle_t h, int n, int d, int l, int q, const double* X
     DH_CHECK_STATUS(pack_omega<false>(h, n, l, Omega, ldo, y, Op));
     DH_CHECK_STATUS(project_fp8(h, n, n, d, l, p, p, c2, Op, y, Q, ldq));
     DH_CHECK_STATUS(dh_range_orth(h, n, l, Q, ldq));
+    // Power iterations alternate the basis between Q and W (no copy for even q).
+    double *cur = Q, *nxt = W;
+    int ldc = ldq, ldn = ldw;
     for (int it = 0; it < q; ++it) {
-        DH_CHECK_STATUS(pack_omega<false>(h, n, l, Q, ldq, y, Op));
-        DH_CHECK_STATUS(project_fp8(h, n, n, d, l, p, p, c2, Op, y, W, ldw));
-        DH_CHECK_STATUS(dh_range_orth(h, n, l, W, ldw));
-        DH_CHECK_HIP(hipMemcpy2DAsync(Q, sizeof(double) * (size_t)ldq, W,
-                                      sizeof(double) * (size_t)ldw, sizeof(double) * (size_t)n, l,
-                                      hipMemcpyDeviceToDevice, h->stream));
+        DH_CHECK_STATUS(pack_omega<false>(h, n, l, cur, ldc, y, Op));
+        DH_CHECK_STATUS(project_fp8(h, n, n, d, l, p, p, c2, Op, y, nxt, ldn));
+        DH_CHECK_STATUS(dh_range_orth(h, n, l, nxt, ldn));
+        std::swap(cur, nxt);
+        std::swap(ldc, ldn);
     }
+    if (cur != Q)
+        DH_CHECK_HIP(hipMemcpy2DAsync(Q, sizeof(double) * (size_t)ldq, cur,
+                                      sizeof(double) * (size_t)ldc, sizeof(double) * (size_t)n, l,
+                                      hipMemcpyDeviceToDevice, h->stream));
     DH_CHECK_STATUS(pack_omega<false>(h, n, l, Q, ldq, y, Op));
     return project_fp8(h, n, n, d, l, p, p, c2, Op, y, W, ldw);
 }

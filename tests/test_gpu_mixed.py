@@ -129,12 +129,13 @@ def test_rbf_project_f16_beats_bf16(dev):
     assert eh < eb / 4
 
 
+@pytest.mark.parametrize("q", [1, 2])
 @pytest.mark.parametrize("labels", [False, True])
-def test_lra_mixed_error_close_to_fp64(dev, labels):
+def test_lra_mixed_error_close_to_fp64(dev, labels, q):
     """Same seeded inputs through the FP64 and the mixed LRA: the mixed
     approximation error stays within a small factor of the FP64 one."""
     import torch
-    n, d, l, q = 4096, 64, 74, 1
+    n, d, l = 4096, 64, 74
     X, y = dh.fill_mixture(5, d, n, 4, 0.3, labels=True)
     y = y if labels else None
     Om = dh.fill_normal(6, n, l)

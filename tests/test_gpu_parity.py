@@ -345,10 +345,12 @@ def test_cholqr_shifted_fallback_ill_conditioned(dev):
 
 # ------------------------------------------------------------------ LRA
 
+@pytest.mark.parametrize("q", [0, 1, 2])
 @pytest.mark.parametrize("labels", [False, True])
-def test_lra_f64_vs_oracle(dev, labels):
+def test_lra_f64_vs_oracle(dev, labels, q):
+    # q = 2: the power iterations alternate the basis between Q and W and end in Q.
     import torch
-    n, d, l, q = 1536, 64, 40, 1
+    n, d, l = 1536, 64, 40
     X = dh.fill_uniform(0, d, n).T.copy()  # DArray set_uniform layout, points as rows
     Om = dh.fill_normal(1, n, l)
     y = np.where(np.arange(n) % 3 == 0, -1.0, 1.0) if labels else None
