@@ -164,15 +164,41 @@ __device__ __forceinline__ void pack_one(long idx, int n, int l, int nchunks, in
     int t = (int)(idx / ncols), c = (int)(idx % ncols);
     int cb = c >> 4, li = c & 15;
     float v[32];
+    const int j0 = 32 * t;
+    const double* src = O + j0 + (size_t)min(c, l - 1) * ldo;
+    const bool vec = c < l && j0 + 32 <= n && ((reinterpret_cast<uintptr_t>(src) & 15) == 0) &&
+                     (!y || (reinterpret_cast<uintptr_t>(y + j0) & 15) == 0);
+    if (vec) {
+        // Whole chunk in range and 16-byte aligned: 16 unconditional 2-double
+        // loads (loads under per-element branches would be issued one by one).
+        double o[32];
 #pragma unroll
-    for (int jj = 0; jj < 32; ++jj) {
-        int j = 32 * t + jj;
-        double o = 0.0;
-        if (j < n && c < l) {
-            o = O[j + (size_t)c * ldo];
-            if (y) o *= y[j];
+        for (int q = 0; q < 16; ++q) {
+            const double2 w = reinterpret_cast<const double2*>(src)[q];
+            o[2 * q] = w.x;
+            o[2 * q + 1] = w.y;
         }
-        v[jj] = (float)o;
+        if (y) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const double2 w = reinterpret_cast<const double2*>(y + j0)[q];
+                o[2 * q] *= w.x;
+                o[2 * q + 1] *= w.y;
+            }
+        }
+#pragma unroll
+        for (int jj = 0; jj < 32; ++jj) v[jj] = (float)o[jj];
+    } else {
+#pragma unroll
+        for (int jj = 0; jj < 32; ++jj) {
+            int j = j0 + jj;
+            double o = 0.0;
+            if (j < n && c < l) {
+                o = O[j + (size_t)c * ldo];
+                if (y) o *= y[j];
+            }
+            v[jj] = (float)o;
+        }
     }
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
