@@ -93,14 +93,36 @@ __global__ __launch_bounds__(256) void maxabs_centred_kernel(int n, int d,
                                                              const double* __restrict__ mean,
                                                              unsigned* __restrict__ maxabs) {
     float mx = 0.f;
-    const long total = (long)n * d;
-    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
-        int f = (int)(e % d), i = (int)(e / d);
-        mx = fmaxf(mx, (float)fabs(X[f + (size_t)i * ldx] - mean[f]));
+    if (d <= 256) {
+        // Thread t keeps feature t % d of row offset t / d (256 / d rows per
+        // block step; no per-element index division), four loads in flight.
+        const int rpi = 256 / d, r = threadIdx.x / d, f = threadIdx.x % d;
+        if (r < rpi) {
+            const double mf = mean[f];
+            const int step = rpi * gridDim.x;
+            int i = blockIdx.x * rpi + r;
+            for (; i + 3 * step < n; i += 4 * step) {
+                const double v0 = X[f + (size_t)i * ldx], v1 = X[f + (size_t)(i + step) * ldx];
+                const double v2 = X[f + (size_t)(i + 2 * step) * ldx];
+                const double v3 = X[f + (size_t)(i + 3 * step) * ldx];
+                mx = fmaxf(mx, fmaxf(fmaxf((float)fabs(v0 - mf), (float)fabs(v1 - mf)),
+                                     fmaxf((float)fabs(v2 - mf), (float)fabs(v3 - mf))));
+            }
+            for (; i < n; i += step) mx = fmaxf(mx, (float)fabs(X[f + (size_t)i * ldx] - mf));
+        }
+    } else {
+        for (int i = blockIdx.x; i < n; i += gridDim.x)
+            for (int f = threadIdx.x; f < d; f += 256)
+                mx = fmaxf(mx, (float)fabs(X[f + (size_t)i * ldx] - mean[f]));
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-    if ((threadIdx.x & 63) == 0) atomicMax(maxabs, __float_as_uint(mx));
+    // One atomic per workgroup: same-address atomics serialise in L2.
+    __shared__ float wmax[4];
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicMax(maxabs, __float_as_uint(fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]))));
 }
 
 __global__ void zero_word_kernel(unsigned* w) { *w = 0u; }
@@ -351,11 +373,11 @@ int prepare_points(dh_handle_t h, int m, int n, int d, const double* Xi, int ldx
         unsigned* word = (unsigned*)(mean + d + ((8 - d % 8) % 8));  // 64-byte tail of the slot
         zero_word_kernel<<<1, 1, 0, h->stream>>>(word);
         DH_CHECK_LAUNCH();
-        int blocks = max(1, min(dh_cdiv((long)n * d, 256), 4 * h->num_cu));
+        int blocks = max(1, min(dh_cdiv((long)n * d, 256), h->num_cu));
         maxabs_centred_kernel<<<blocks, 256, 0, h->stream>>>(n, d, Xj, ldxj, mean, word);
         DH_CHECK_LAUNCH();
         if (!same) {
-            blocks = max(1, min(dh_cdiv((long)m * d, 256), 4 * h->num_cu));
+            blocks = max(1, min(dh_cdiv((long)m * d, 256), h->num_cu));
             maxabs_centred_kernel<<<blocks, 256, 0, h->stream>>>(m, d, Xi, ldxi, mean, word);
             DH_CHECK_LAUNCH();
         }
