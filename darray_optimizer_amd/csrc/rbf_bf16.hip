@@ -19,6 +19,31 @@ __global__ __launch_bounds__(256) void colsum_partial_kernel(int n, int d, const
                                                              double* __restrict__ partial) {
     int b = blockIdx.x;
     int i1 = min(n, (b + 1) * slice);
+    if (d <= 128) {
+        // 256 / d row groups (thread t: feature t % d, rows t / d + k * rpi),
+        // four sums each, combined through LDS in a fixed order.
+        __shared__ double red[256];
+        const int rpi = 256 / d, r = threadIdx.x / d, f = threadIdx.x % d;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        if (r < rpi) {
+            int i = b * slice + r;
+            for (; i + 3 * rpi < i1; i += 4 * rpi) {
+                s0 += X[f + (size_t)i * ldx];
+                s1 += X[f + (size_t)(i + rpi) * ldx];
+                s2 += X[f + (size_t)(i + 2 * rpi) * ldx];
+                s3 += X[f + (size_t)(i + 3 * rpi) * ldx];
+            }
+            for (; i < i1; i += rpi) s0 += X[f + (size_t)i * ldx];
+        }
+        red[threadIdx.x] = (s0 + s1) + (s2 + s3);
+        __syncthreads();
+        if (threadIdx.x < d) {
+            double t = 0.0;
+            for (int gq = 0; gq < rpi; ++gq) t += red[gq * d + threadIdx.x];
+            partial[(size_t)b * d + threadIdx.x] = t;
+        }
+        return;
+    }
     for (int f = threadIdx.x; f < d; f += 256) {
         // Four independent sums so that the loads of consecutive points
         // overlap (fixed order: deterministic).
