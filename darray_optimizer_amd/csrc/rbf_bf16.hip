@@ -60,10 +60,35 @@ __global__ __launch_bounds__(256) void colsum_partial_kernel(int n, int d, const
     }
 }
 
-__global__ __launch_bounds__(256) void colmean_final_kernel(int n, int d, int nb,
-                                                            const double* __restrict__ partial,
-                                                            double* __restrict__ mean) {
-    for (int f = blockIdx.x * 256 + threadIdx.x; f < d; f += gridDim.x * 256) {
+__global__ __launch_bounds__(1024) void colmean_final_kernel(int n, int d, int nb,
+                                                             const double* __restrict__ partial,
+                                                             double* __restrict__ mean) {
+    if (gridDim.x == 1 && d <= (int)blockDim.x) {
+        // One workgroup: blockDim / d slab groups per feature, four sums each,
+        // combined through LDS in a fixed order.
+        __shared__ double red[1024];
+        const int rpi = blockDim.x / d, r = threadIdx.x / d, f = threadIdx.x % d;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        if (r < rpi) {
+            int b = r;
+            for (; b + 3 * rpi < nb; b += 4 * rpi) {
+                s0 += partial[(size_t)b * d + f];
+                s1 += partial[(size_t)(b + rpi) * d + f];
+                s2 += partial[(size_t)(b + 2 * rpi) * d + f];
+                s3 += partial[(size_t)(b + 3 * rpi) * d + f];
+            }
+            for (; b < nb; b += rpi) s0 += partial[(size_t)b * d + f];
+        }
+        red[threadIdx.x] = (s0 + s1) + (s2 + s3);
+        __syncthreads();
+        if ((int)threadIdx.x < d) {
+            double t = 0.0;
+            for (int gq = 0; gq < rpi; ++gq) t += red[gq * d + threadIdx.x];
+            mean[threadIdx.x] = n > 0 ? t / n : 0.0;
+        }
+        return;
+    }
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < d; f += gridDim.x * blockDim.x) {
         double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
         int b = 0;
         for (; b + 4 <= nb; b += 4) {
@@ -288,7 +313,10 @@ int point_mean(dh_handle_t h, int n, int d, const double* X, int ldx, double* me
     if (!part) return DH_ERR_ALLOC;
     colsum_partial_kernel<<<nb, 256, 0, h->stream>>>(n, d, X, ldx, slice, part);
     DH_CHECK_LAUNCH();
-    colmean_final_kernel<<<dh_cdiv(d, 256), 256, 0, h->stream>>>(n, d, nb, part, mean);
+    if (d <= 256)
+        colmean_final_kernel<<<1, 1024, 0, h->stream>>>(n, d, nb, part, mean);
+    else
+        colmean_final_kernel<<<dh_cdiv(d, 256), 256, 0, h->stream>>>(n, d, nb, part, mean);
     DH_CHECK_LAUNCH();
     return 0;
 }
