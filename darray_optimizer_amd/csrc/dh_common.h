@@ -43,10 +43,17 @@ struct dh_handle_s {
     // the blocked Cholesky; created on first use, released by dh_destroy.
     hipStream_t aux = nullptr;
     hipEvent_t ev_panel = nullptr, ev_update = nullptr;
+    // Pinned host words for device status reads (created on first use).
+    int* host_status = nullptr;
 };
 
 // Creates h->aux and its events if they do not exist yet.
 int dh_aux_stream(dh_handle_t h);
+
+// Pinned host buffer of 16 ints for status words copied back from the
+// device (a copy into pageable memory takes a slower staged path); nullptr
+// if it cannot be allocated.
+int* dh_host_status(dh_handle_t h);
 
 // Bracket one fused projection kernel launch with timing events (no-ops
 // unless dh_timing_enable switched timing on for the handle).

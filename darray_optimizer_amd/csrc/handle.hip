@@ -95,9 +95,17 @@ extern "C" int dh_destroy(dh_handle_t h) {
     if (h->ev_panel) (void)hipEventDestroy(h->ev_panel);
     if (h->ev_update) (void)hipEventDestroy(h->ev_update);
     if (h->aux) (void)hipStreamDestroy(h->aux);
+    if (h->host_status) (void)hipHostFree(h->host_status);
     if (h->own_stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return 0;
+}
+
+int* dh_host_status(dh_handle_t h) {
+    if (!h->host_status &&
+        hipHostMalloc((void**)&h->host_status, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess)
+        h->host_status = nullptr;
+    return h->host_status;
 }
 
 int dh_aux_stream(dh_handle_t h) {

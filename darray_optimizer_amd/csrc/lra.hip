@@ -224,8 +224,10 @@ int dh_cholqr2_impl(dh_handle_t h, int m, int n, double* Y, int ldy, double* R, 
     if (R) DH_CHECK_STATUS(accumulate_r(h, n, G, Racc, Rtmp, true, nullptr, ldr));
     DH_CHECK_STATUS(cholqr_sweep_small_async(h, m, n, Y, ldy, G, Dbuf, dinfo, 1));
     if (R) DH_CHECK_STATUS(accumulate_r(h, n, G, Racc, Rtmp, false, R, ldr));
-    int hinfo[2] = {0, 0};
-    DH_CHECK_HIP(hipMemcpyAsync(hinfo, dinfo, sizeof(hinfo), hipMemcpyDeviceToHost, h->stream));
+    int hbuf[2] = {0, 0};
+    int* pinned = dh_host_status(h);
+    int* hinfo = pinned ? pinned : hbuf;
+    DH_CHECK_HIP(hipMemcpyAsync(hinfo, dinfo, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
     DH_CHECK_HIP(hipStreamSynchronize(h->stream));
     if (hinfo[0] == 0) {
         if (hinfo[1] == 0) return 0;
