@@ -434,9 +434,12 @@ int prepare_points(dh_handle_t h, int m, int n, int d, const double* Xi, int ldx
             maxabs_centred_kernel<<<blocks, 256, 0, h->stream>>>(m, d, Xi, ldxi, mean, word);
             DH_CHECK_LAUNCH();
         }
-        unsigned bits = 0;
-        DH_CHECK_HIP(hipMemcpyAsync(&bits, word, sizeof(bits), hipMemcpyDeviceToHost, h->stream));
+        unsigned hbits = 0;
+        unsigned* hb = (unsigned*)dh_host_status(h);  // pinned when available
+        if (!hb) hb = &hbits;
+        DH_CHECK_HIP(hipMemcpyAsync(hb, word, sizeof(unsigned), hipMemcpyDeviceToHost, h->stream));
         DH_CHECK_HIP(hipStreamSynchronize(h->stream));
+        const unsigned bits = *hb;
         float mx;
         std::memcpy(&mx, &bits, sizeof(mx));
         if (mx > 0.f && std::isfinite(mx)) {
