@@ -17,7 +17,8 @@ from . import _capi
 from ._capi import DH_PREC_BF16, DH_PREC_F16, DH_PREC_F64, DH_PREC_FP8, DHError, call, lib
 
 __all__ = ["Handle", "DHError", "DH_PREC_F64", "DH_PREC_BF16", "DH_PREC_F16", "DH_PREC_FP8", "lib", "fill_uniform",
-           "fill_normal", "fill_mixture", "mt19937_raw", "to_dev", "from_dev"]
+           "fill_normal", "fill_mixture", "mt19937_raw", "to_dev", "from_dev", "read_libsvm",
+           "write_libsvm_model"]
 
 
 def _ptr(t):
@@ -216,6 +217,22 @@ class Handle:
         r = call("dh_lra_batched", self.h, prec, len(problems), arr, st)
         return r, list(st)[:len(problems)]
 
+    def gesvj(self, A, sva, V):
+        """One-sided Jacobi SVD in place: A (n, m) tensor <- U diag(sva); returns
+        (status, sweeps)."""
+        m, n = A.shape[1], A.shape[0]
+        sw = C.c_int(0)
+        st = call("dh_gesvj_f64", self.h, m, n, _ptr(A), _ld(A), _ptr(sva), _ptr(V), _ld(V),
+                  C.byref(sw))
+        return st, sw.value
+
+    def nystrom(self, Q, W, k, Z, lam=None) -> int:
+        """Z (n x k) with Z Z^T the rank-k truncation of Q Q^T K Q Q^T, from Q and
+        W = K Q ((l, n) tensors); lam (k) receives the eigenvalues of Q^T K Q."""
+        n, l = Q.shape[1], Q.shape[0]
+        return call("dh_nystrom_f64", self.h, n, l, k, _ptr(Q), _ld(Q), _ptr(W), _ld(W), _ptr(Z),
+                    _ld(Z), _ptr(lam))
+
     def lra_error(self, X, gamma, Q, W, y=None):
         """(||K - Q W^T||_F^2, ||K||_F^2) in FP64."""
         n, d = X.shape
@@ -259,3 +276,28 @@ def fill_mixture(seed: int, d: int, n: int, ncentres: int, sigma: float, labels=
     call("dh_fill_mixture", seed, d, n, ncentres, sigma, _dptr(X), d,
          _dptr(y) if labels else None, None)
     return (X, y) if labels else X
+
+
+# ---------------------------------------------------------- LIBSVM files
+
+def read_libsvm(path, n: int, d: int):
+    """(X (n, d) points as rows, y (n), points read) from a LIBSVM data file."""
+    X = np.empty((n, d), dtype=np.float64)  # = d x n column-major, ld d
+    y = np.empty(n, dtype=np.float64)
+    got = call("dh_read_libsvm", str(path).encode(), n, d, _dptr(X), max(1, d), _dptr(y))
+    return X, y, got
+
+
+def write_libsvm_model(path, a, C_, Z, y, X, gamma):
+    """Write the LIBSVM model of the dual solution a; Z is (n, k) (Nystrom
+    factor rows), X (n, d) points as rows.  Returns (support vectors, bias)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    Zf = np.asfortranarray(Z, dtype=np.float64)
+    Xp = np.ascontiguousarray(X, dtype=np.float64)
+    n, k = Zf.shape
+    d = Xp.shape[1]
+    b = C.c_double(0.0)
+    nsv = call("dh_write_libsvm_model", str(path).encode(), n, d, k, _dptr(a), float(C_), _dptr(Zf),
+               max(1, n), _dptr(y), _dptr(Xp), max(1, d), float(gamma), C.byref(b))
+    return nsv, b.value

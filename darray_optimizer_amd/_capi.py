@@ -61,6 +61,11 @@ SIGNATURES: dict[str, tuple] = {
     "dh_lra": (_i, [_p, _i, _i, _i, _i, _i, _p, _i, _p, _d, _p, _i, _p, _i, _p, _i]),
     "dh_lra_batched": (_i, [_p, _i, _i, C.POINTER(LraProblem), C.POINTER(_i)]),
     "dh_lra_error": (_i, [_p, _i, _i, _i, _p, _i, _p, _d, _p, _i, _p, _i, C.POINTER(_d)]),
+    "dh_gesvj_f64": (_i, [_p, _i, _i, _p, _i, _p, _p, _i, C.POINTER(_i)]),
+    "dh_nystrom_f64": (_i, [_p, _i, _i, _i, _p, _i, _p, _i, _p, _i, _p]),
+    "dh_read_libsvm": (_i, [C.c_char_p, _i, _i, C.POINTER(_d), _i, C.POINTER(_d)]),
+    "dh_write_libsvm_model": (_i, [C.c_char_p, _i, _i, _i, C.POINTER(_d), _d, C.POINTER(_d), _i,
+                                   C.POINTER(_d), C.POINTER(_d), _i, _d, C.POINTER(_d)]),
     "dh_mt19937_raw": (_i, [_u32, _i, C.POINTER(_u32)]),
     "dh_fill_uniform": (_i, [_u32, _d, _d, _i, _i, C.POINTER(_d), _i]),
     "dh_fill_normal": (_i, [_u32, _d, _d, _i, _i, C.POINTER(_d), _i]),
@@ -75,6 +80,7 @@ DH_PREC_FP8 = 3
 DH_ERR_HIP = -1000
 DH_ERR_ALLOC = -1001
 DH_ERR_UNSUPPORTED = -1002
+DH_ERR_NOT_CONVERGED = -1003
 
 _lib = None
 

@@ -141,9 +141,19 @@ struct Carve {
     }
 };
 
+// Field by field: callers that fill the struct member by member leave its
+// padding bytes undefined, so a byte comparison would miss equal tables.
+bool same_problem(const dh_lra_problem& a, const dh_lra_problem& b) {
+    return a.n == b.n && a.d == b.d && a.l == b.l && a.q == b.q && a.X == b.X && a.ldx == b.ldx &&
+           a.y == b.y && a.gamma == b.gamma && a.Omega == b.Omega && a.ldo == b.ldo &&
+           a.Q == b.Q && a.ldq == b.ldq && a.W == b.W && a.ldw == b.ldw;
+}
+
 bool same_problems(const Plan& p, int prec, int count, const dh_lra_problem* probs) {
-    return p.exec && p.prec == prec && (int)p.probs.size() == count &&
-           std::memcmp(p.probs.data(), probs, sizeof(dh_lra_problem) * count) == 0;
+    if (!p.exec || p.prec != prec || (int)p.probs.size() != count) return false;
+    for (int i = 0; i < count; ++i)
+        if (!same_problem(p.probs[i], probs[i])) return false;
+    return true;
 }
 
 }  // namespace

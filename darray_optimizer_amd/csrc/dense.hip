@@ -103,7 +103,11 @@ static int launch_gemm(dh_handle_t h, bool ta, bool tb, int m, int n, int k, T a
     if (nsplit < 1) nsplit = 1;
     T* part = nullptr;
     if (nsplit > 1) {
-        part = (T*)dh_ws(h, DH_WS_SPLITK, sizeof(T) * (size_t)nsplit * m * n);
+        // GEMMs queued on the look-ahead stream run concurrently with those of
+        // the handle's stream: each stream has its own partials slot (a slot
+        // only grows after the stream that uses it has drained).
+        const dh_slot slot = (h->aux && h->stream == h->aux) ? DH_WS_SPLITK_AUX : DH_WS_SPLITK;
+        part = (T*)dh_ws(h, slot, sizeof(T) * (size_t)nsplit * m * n);
         if (!part) return DH_ERR_ALLOC;
     }
     dim3 grid(tm, tn, nsplit);

@@ -13,6 +13,7 @@
 // same time never share a slot.
 enum dh_slot {
     DH_WS_SPLITK = 0,  // split-K partial products
+    DH_WS_SPLITK_AUX,  // split-K partial products of GEMMs on the look-ahead stream
     DH_WS_NORMS,       // squared norms of point sets
     DH_WS_GRAM,        // CholeskyQR Gram matrix / R factors
     DH_WS_RBUF,        // second R factor
@@ -26,6 +27,7 @@ enum dh_slot {
     DH_WS_SOLVE_R,     // residual / correction panels of iterative refinement
     DH_WS_TRSM_INV,    // inverses of the 64 x 64 diagonal blocks of a triangular solve
     DH_WS_TRSM_X,      // solution blocks of the block-sweep triangular solve
+    DH_WS_EIG,         // Jacobi SVD / Nystrom work matrices
     DH_WS_COUNT
 };
 
@@ -45,10 +47,17 @@ struct dh_handle_s {
     hipEvent_t ev_panel = nullptr, ev_update = nullptr;
     // Pinned host words for device status reads (created on first use).
     int* host_status = nullptr;
+    // Resident workgroups per CU of the persistent kernels, by kernel
+    // (queried on the handle's device on first use; handle.hip).
+    void* occupancy = nullptr;
 };
 
 // Creates h->aux and its events if they do not exist yet.
 int dh_aux_stream(dh_handle_t h);
+
+// *per_cu = resident workgroups per CU of `kernel` at `threads` threads per
+// workgroup on the handle's device (at least 1), cached in the handle.
+int dh_occupancy(dh_handle_t h, const void* kernel, int threads, int* per_cu);
 
 // Pinned host buffer of 16 ints for status words copied back from the
 // device (a copy into pageable memory takes a slower staged path); nullptr

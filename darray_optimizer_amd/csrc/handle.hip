@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <new>
 
 #include <utility>
@@ -96,6 +97,7 @@ extern "C" int dh_destroy(dh_handle_t h) {
     if (h->ev_update) (void)hipEventDestroy(h->ev_update);
     if (h->aux) (void)hipStreamDestroy(h->aux);
     if (h->host_status) (void)hipHostFree(h->host_status);
+    delete static_cast<std::map<std::pair<const void*, int>, int>*>(h->occupancy);
     if (h->own_stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return 0;
@@ -106,6 +108,23 @@ int* dh_host_status(dh_handle_t h) {
         hipHostMalloc((void**)&h->host_status, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         h->host_status = nullptr;
     return h->host_status;
+}
+
+int dh_occupancy(dh_handle_t h, const void* kernel, int threads, int* per_cu) {
+    using Table = std::map<std::pair<const void*, int>, int>;
+    if (!h->occupancy) h->occupancy = new Table();
+    Table& t = *static_cast<Table*>(h->occupancy);
+    auto it = t.find({kernel, threads});
+    if (it != t.end()) {
+        *per_cu = it->second;
+        return 0;
+    }
+    int v = 0;
+    DH_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, kernel, threads, 0));
+    v = v < 1 ? 1 : v;
+    t[{kernel, threads}] = v;
+    *per_cu = v;
+    return 0;
 }
 
 int dh_aux_stream(dh_handle_t h) {

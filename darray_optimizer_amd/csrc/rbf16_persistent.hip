@@ -31,11 +31,8 @@ template <int DK, int NCB, int W, int RB>
 int launch_persistent(dh_handle_t h, const DhProjJob* jobs, const DhProjItem* items, int nitems,
                       int* counter) {
     auto kern = rbf_project_bf16_persistent<DK, NCB, W, RB>;
-    static int per_cu = 0;  // resident workgroups per CU, queried once
-    if (per_cu == 0) {
-        DH_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, W * 64, 0));
-        per_cu = max(per_cu, 1);
-    }
+    int per_cu = 1;
+    DH_CHECK_STATUS(dh_occupancy(h, (const void*)kern, W * 64, &per_cu));
     int grid = max(1, min(nitems, per_cu * h->num_cu));
     kern<<<grid, W * 64, 0, h->stream>>>(jobs, items, nitems, counter);
     DH_CHECK_LAUNCH();

@@ -161,38 +161,13 @@ extern "C" int dh_posv_mixed(dh_handle_t h, char uplo, int n, int nrhs, const do
     double* nrm = (double*)(base + nrm_off);  // b2[nrhs], x2[nrhs], r2[nrhs], scratch...
     double *b2 = nrm, *x2 = nrm + nrhs, *r2 = nrm + 2 * nrhs, *scr = nrm + 3 * nrhs;
 
-    d2s_upper_kernel<<<dh_cdiv((long)nn, 256), 256, 0, h->stream>>>(n, A, lda, S, n);
-    DH_CHECK_LAUNCH();
-    int info = 0;
-    DH_CHECK_STATUS(dh_potrf_impl<float>(h, n, S, n, bs, &info));
-    if (info) {
-        // FP32 factorisation failed: FP64 solve on a copy (dsposv falls back likewise).
-        double* A64 = (double*)dh_ws(h, DH_WS_SOLVE_A, sizeof(double) * nn);
-        if (!A64) return DH_ERR_ALLOC;
-        copy_kernel<<<dh_cdiv((long)nn, 256), 256, 0, h->stream>>>(n, n, A, lda, A64, n);
-        DH_CHECK_LAUNCH();
-        copy_kernel<<<dh_cdiv((long)nr, 256), 256, 0, h->stream>>>(n, nrhs, B, ldb, X, ldx);
-        DH_CHECK_LAUNCH();
-        DH_CHECK_STATUS(dh_potrf_impl<double>(h, n, A64, n, bs, &info));
-        if (info) return dh_fail(info, "%s: matrix not positive definite (minor %d)", name, info);
-        DH_CHECK_STATUS(two_solves<double>(h, n, nrhs, A64, n, X, ldx));
-        DH_CHECK_HIP(hipStreamSynchronize(h->stream));
-        if (iters) *iters = -1;
-        return 0;
-    }
     DH_CHECK_STATUS(launch_fro2_sym(h, n, A, lda, scr, scr + 1000));
     colnorm2_kernel<<<nrhs, 256, 0, h->stream>>>(n, B, ldb, b2);
     DH_CHECK_LAUNCH();
-    // X0 = A32^{-1} B
-    d2s_kernel<<<dh_cdiv((long)nr, 256), 256, 0, h->stream>>>(n, nrhs, B, ldb, Rf, n);
-    DH_CHECK_LAUNCH();
-    DH_CHECK_STATUS(two_solves<float>(h, n, nrhs, S, n, Rf, n));
-    s2d_kernel<<<dh_cdiv((long)nr, 256), 256, 0, h->stream>>>(n, nrhs, Rf, n, X, ldx, 0);
-    DH_CHECK_LAUNCH();
     std::vector<double> hn(3 * nrhs + 1);
-    double be = INFINITY;
-    int it = 0;
-    for (;; ++it) {
+    // Normwise backward error max_j ||b_j - A x_j|| / (||A||_F ||x_j|| + ||b_j||) of X,
+    // leaving the residual in R; +inf when any column's error is not finite.
+    auto backward_error = [&](double* out) -> int {
         copy_kernel<<<dh_cdiv((long)nr, 256), 256, 0, h->stream>>>(n, nrhs, B, ldb, R, n);
         DH_CHECK_LAUNCH();
         DH_CHECK_STATUS(dh_symm_upper_impl<double>(h, n, nrhs, -1.0, A, lda, X, ldx, 1.0, R, n));
@@ -206,14 +181,64 @@ extern "C" int dh_posv_mixed(dh_handle_t h, char uplo, int n, int nrhs, const do
         DH_CHECK_HIP(hipMemcpyAsync(&anorm2, scr + 1000, sizeof(double), hipMemcpyDeviceToHost,
                                     h->stream));
         DH_CHECK_HIP(hipStreamSynchronize(h->stream));
-        double anorm = std::sqrt(anorm2);
-        be = 0.0;
+        const double anorm = std::sqrt(anorm2);
+        double be = 0.0;
         for (int j = 0; j < nrhs; ++j) {
             double den = anorm * std::sqrt(hn[nrhs + j]) + std::sqrt(hn[j]);
             double e = den > 0.0 ? std::sqrt(hn[2 * nrhs + j]) / den : 0.0;
+            if (!std::isfinite(e)) e = INFINITY;  // NaN must not compare as converged
             be = std::max(be, e);
         }
-        if (!(be > tol) || it >= max_iter) break;
+        *out = be;
+        return 0;
+    };
+    // FP64 Cholesky solve on a copy of A, as LAPACK dsposv falls back when the
+    // FP32 factorisation breaks down or the refinement does not converge.
+    auto fp64_solve = [&]() -> int {
+        double* A64 = (double*)dh_ws(h, DH_WS_SOLVE_A, sizeof(double) * nn);
+        if (!A64) return DH_ERR_ALLOC;
+        copy_kernel<<<dh_cdiv((long)nn, 256), 256, 0, h->stream>>>(n, n, A, lda, A64, n);
+        DH_CHECK_LAUNCH();
+        copy_kernel<<<dh_cdiv((long)nr, 256), 256, 0, h->stream>>>(n, nrhs, B, ldb, X, ldx);
+        DH_CHECK_LAUNCH();
+        int info64 = 0;
+        DH_CHECK_STATUS(dh_potrf_impl<double>(h, n, A64, n, bs, &info64));
+        if (info64)
+            return dh_fail(info64, "%s: matrix not positive definite (minor %d)", name, info64);
+        DH_CHECK_STATUS(two_solves<double>(h, n, nrhs, A64, n, X, ldx));
+        double be = 0.0;
+        DH_CHECK_STATUS(backward_error(&be));
+        if (iters) *iters = -1;
+        if (berr) *berr = be;
+        if (!(be <= tol) && !(be <= 1e3 * n * 1.1102230246251565e-16))
+            return dh_fail(DH_ERR_NOT_CONVERGED,
+                           "%s: FP64 fallback solve has backward error %.3e (tol %.3e)", name, be,
+                           tol);
+        return 0;
+    };
+
+    d2s_upper_kernel<<<dh_cdiv((long)nn, 256), 256, 0, h->stream>>>(n, A, lda, S, n);
+    DH_CHECK_LAUNCH();
+    int info = 0;
+    DH_CHECK_STATUS(dh_potrf_impl<float>(h, n, S, n, bs, &info));
+    if (info) return fp64_solve();  // FP32 factorisation broke down
+    // X0 = A32^{-1} B
+    d2s_kernel<<<dh_cdiv((long)nr, 256), 256, 0, h->stream>>>(n, nrhs, B, ldb, Rf, n);
+    DH_CHECK_LAUNCH();
+    DH_CHECK_STATUS(two_solves<float>(h, n, nrhs, S, n, Rf, n));
+    s2d_kernel<<<dh_cdiv((long)nr, 256), 256, 0, h->stream>>>(n, nrhs, Rf, n, X, ldx, 0);
+    DH_CHECK_LAUNCH();
+    double be = INFINITY, prev = INFINITY;
+    int it = 0;
+    for (;; ++it) {
+        DH_CHECK_STATUS(backward_error(&be));
+        if (be <= tol) break;
+        // Not converged within max_iter corrections, diverged (non-finite), or
+        // stalled (a correction that does not halve the error: kappa * u32
+        // near 1): solve in FP64 instead.
+        if (it >= max_iter || !std::isfinite(be) || (it > 0 && be > 0.5 * prev))
+            return fp64_solve();
+        prev = be;
         d2s_kernel<<<dh_cdiv((long)nr, 256), 256, 0, h->stream>>>(n, nrhs, R, n, Rf, n);
         DH_CHECK_LAUNCH();
         DH_CHECK_STATUS(two_solves<float>(h, n, nrhs, S, n, Rf, n));

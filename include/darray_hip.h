@@ -33,6 +33,7 @@ extern "C" {
 #define DH_ERR_HIP (-1000)         /* a HIP runtime call failed */
 #define DH_ERR_ALLOC (-1001)       /* device allocation failed */
 #define DH_ERR_UNSUPPORTED (-1002) /* shape outside what the kernels handle */
+#define DH_ERR_NOT_CONVERGED (-1003) /* an iterative solve missed its tolerance */
 
 typedef struct dh_handle_s* dh_handle_t;
 
@@ -184,10 +185,14 @@ int dh_posv_f64(dh_handle_t h, char uplo, int n, int nrhs, double* A, int lda, d
 /* Mixed-precision A X = B: FP32 Cholesky (block bs) of a rounded copy of A,
  * FP64 iterative refinement with residuals R = B - A X computed from the
  * upper triangle of A in FP64, until the normwise backward error
- * max_j ||r_j||_2 / (||A||_F ||x_j||_2 + ||b_j||_2) <= tol or max_iter
- * corrections.  A and B are not modified.  *iters = corrections applied
- * (-1: the FP32 factorisation broke down and the FP64 path was used, as in
- * LAPACK dsposv); *berr = final backward error.  Synchronises. */
+ * max_j ||r_j||_2 / (||A||_F ||x_j||_2 + ||b_j||_2) <= tol.  As LAPACK dsposv,
+ * the call solves in FP64 instead (*iters = -1) when the FP32 factorisation
+ * breaks down, or the refinement does not reach tol within max_iter
+ * corrections, stalls (a correction that does not halve the error) or
+ * produces a non-finite error.  A and B are not modified.  *iters =
+ * corrections applied; *berr = backward error of the returned X.  Returns
+ * DH_ERR_NOT_CONVERGED when even the FP64 solution misses tol and
+ * 1e3 * n * eps64 (non-finite input).  Synchronises. */
 int dh_posv_mixed(dh_handle_t h, char uplo, int n, int nrhs, const double* A, int lda,
                   const double* B, int ldb, double* X, int ldx, int bs, double tol, int max_iter,
                   int* iters, double* berr);
@@ -264,6 +269,58 @@ int dh_lra_batched(dh_handle_t h, int prec, int count, const dh_lra_problem* pro
 int dh_lra_error(dh_handle_t h, int n, int d, int l, const double* X, int ldx, const double* y,
                  double gamma, const double* Q, int ldq, const double* W, int ldw,
                  double* err2);
+
+/* ---------------------------------------------------- Nystrom factor */
+
+/* One-sided (Hestenes) Jacobi SVD in FP64 of A (m x n, m >= n): column pairs
+ * are rotated to orthogonality in round-robin order until a sweep rotates no
+ * pair (|a_p.a_q| <= 1e-14 |a_p| |a_q| for all pairs).  On exit A = U diag(sva)
+ * (columns not normalised), sva (n, DEVICE) the singular values in descending
+ * order and V (n x n) the right singular vectors, columns in the same order;
+ * *sweeps (host, may be NULL) the sweeps used.  Returns 0, or 1 if 60 sweeps
+ * did not converge.  Replaces sgesvd_ in DArray::svd (lapack_like.h:388-413).
+ * Synchronises. */
+int dh_gesvj_f64(dh_handle_t h, int m, int n, double* A, int lda, double* sva, double* V,
+                 int ldv, int* sweeps);
+
+/* Nystrom factor of a range finder's output (Q n x l orthonormal, W = K Q as
+ * dh_lra returns them): C = (Q^T W + W^T Q)/2 = Q^T K Q, C = V diag(lam) V^T
+ * by dh_gesvj_f64 (lam_i = v_i.(C v_i)), and Z (n x k) = Q V_k
+ * diag(sqrt(max(lam_k, 0))) for the k largest eigenvalues, so that
+ * Z Z^T ~ K (its error through dh_lra_error with Q = W = Z).  lambda (k,
+ * DEVICE, may be NULL) receives those eigenvalues in descending order.  The
+ * reference forms C with tc_gemm and takes Z = Q diag(S) from sgesvd_'s
+ * singular values alone (testsvm.cpp:126-172).  Returns 0, or 1 if the
+ * Jacobi iteration did not converge.  Synchronises. */
+int dh_nystrom_f64(dh_handle_t h, int n, int l, int k, const double* Q, int ldq, const double* W,
+                   int ldw, double* Z, int ldz, double* lambda);
+
+/* ------------------------------------------------------ LIBSVM files */
+/* Host functions (HOST pointers; no GPU use). */
+
+/* Read n points of a LIBSVM data file (read_input_file, read_input.h:8-38):
+ * line j is "<label> <index>:<value> ..." with 1-based feature indices; X
+ * (d x n, one point per column, ld ldx) and y (n) are zeroed, then
+ * X[index-1, j] = value and y[j] = +1 if the label is 1, else -1 (the
+ * reference's rule).  Indices <= 0 are skipped as in the reference; an index
+ * above d returns DH_ERR_UNSUPPORTED (the reference writes past the column)
+ * and a token that is not index:value returns -1.  Returns the number of
+ * points read (< n when the file has fewer lines). */
+int dh_read_libsvm(const char* path, int n, int d, double* X, int ldx, double* y);
+
+/* Write a LIBSVM "c_svc" / "rbf" model file for the dual solution a (n) of
+ * the box-constrained SVM with bound C (writemodel, dbarrier.h:394-470):
+ * support vectors are the points with a_i > 1e-6, the bias b is the mean of
+ * y_j (1 - sum_i a_i (Z Z^T)_ij) over the first (up to) 50 free support
+ * vectors j (1e-6 < a_j < C - 1e-6), Z (n x k, ld ldz) the Nystrom factor
+ * with Z Z^T ~ diag(y) K diag(y); rho = -b, coefficients y_i a_i, points X
+ * (d x n, ld ldx) written sparsely, labels "1 -1" with the class-+1 support
+ * vectors first (the reference writes class sizes in "label" and uses the
+ * diagonal (Z Z^T)_ii in the bias).  *bias (may be NULL) receives b.
+ * Returns the number of support vectors. */
+int dh_write_libsvm_model(const char* path, int n, int d, int k, const double* a, double C,
+                          const double* Z, int ldz, const double* y, const double* X, int ldx,
+                          double gamma, double* bias);
 
 /* ---------------------------------------------------- host-side inputs */
 /* Seeded host generators (HOST pointers; no GPU use). */

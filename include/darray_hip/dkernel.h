@@ -25,9 +25,18 @@ extern void cuda_finalize();
 
 namespace DArray {
 
-// Arithmetic of the K-passes: FP64 MFMA throughout, or BF16 MFMA with FP32
-// accumulation (DH_PREC_BF16; points centred and rounded once, d <= 256).
-enum class Precision { FP64 = DH_PREC_F64, Mixed = DH_PREC_BF16 };
+// Arithmetic of the K-passes: FP64 MFMA throughout; Mixed = FP16 MFMA with
+// FP32 accumulation (DH_PREC_F16: points centred, scaled by a power of two and
+// rounded once; the mixed mode whose approximation error stays within 1.1x of
+// FP64); BF16 (DH_PREC_BF16, same kernels with 3 fewer mantissa bits); FP8
+// (DH_PREC_FP8: e4m3 distance, BF16 projection).  The 16-bit and FP8 modes
+// take d <= 256.
+enum class Precision {
+    FP64 = DH_PREC_F64,
+    Mixed = DH_PREC_F16,
+    BF16 = DH_PREC_BF16,
+    FP8 = DH_PREC_FP8
+};
 
 inline Precision& lra_precision() {
     static thread_local Precision p = Precision::FP64;
@@ -99,12 +108,24 @@ void LRA(Grid g, int n, int d, int k, DMatrix<T> X, DMatrix<T>& Y, DMatrix<T>& O
     detail::F64In<T> x(X), y(Y), o(O);
     detail::F64Out<T> a(A);
     int st;
-    if (prec == Precision::FP64)
-        st = dh_rbf_project_f64(handle(), n, n, d, k, x.p, x.ld, x.p, x.ld, y.p, y.p, gamma, o.p,
-                                o.ld, 0.0, a.p, a.ld);
-    else
-        st = dh_rbf_project_bf16(handle(), n, n, d, k, x.p, x.ld, x.p, x.ld, y.p, y.p, gamma, o.p,
-                                 o.ld, a.p, a.ld);
+    switch (prec) {
+        case Precision::FP64:
+            st = dh_rbf_project_f64(handle(), n, n, d, k, x.p, x.ld, x.p, x.ld, y.p, y.p, gamma,
+                                    o.p, o.ld, 0.0, a.p, a.ld);
+            break;
+        case Precision::Mixed:
+            st = dh_rbf_project_f16(handle(), n, n, d, k, x.p, x.ld, x.p, x.ld, y.p, y.p, gamma,
+                                    o.p, o.ld, a.p, a.ld);
+            break;
+        case Precision::BF16:
+            st = dh_rbf_project_bf16(handle(), n, n, d, k, x.p, x.ld, x.p, x.ld, y.p, y.p, gamma,
+                                     o.p, o.ld, a.p, a.ld);
+            break;
+        default:
+            st = dh_rbf_project_fp8(handle(), n, n, d, k, x.p, x.ld, x.p, x.ld, y.p, y.p, gamma,
+                                    o.p, o.ld, a.p, a.ld);
+            break;
+    }
     check(st, "LRA");
     a.commit();
 }

@@ -10,6 +10,8 @@
 // (0 on success).
 #pragma once
 
+#include <vector>
+
 #include "blas_like.h"
 
 namespace DArray {
@@ -132,6 +134,53 @@ inline int cholesky_qr2(DMatrix<double>& Y, DMatrix<double>* R = nullptr) {
     int st = dh_cholqr2_f64(handle(), m, n, Y.device_data(), Y.lld(),
                             R ? R->device_data() : nullptr, R ? R->lld() : 1);
     if (st < 0) check(st, "cholesky_qr2");
+    return st;
+}
+
+// Singular values of the square block LA, descending, as an m x 1 LMatrix
+// (DArray::svd, lapack_like.h:388-413, which calls sgesvd_ "A","A" on the
+// host and returns S).  Computed on the device in FP64 by the one-sided
+// Jacobi SVD (dh_gesvj_f64); as in the reference the index arguments are
+// not used.
+template <typename T>
+LMatrix<T> svd(int rank, LMatrix<T> LA, int i1, int j1, int i2, int j2) {
+    (void)rank;
+    (void)i1;
+    (void)j1;
+    (void)i2;
+    (void)j2;
+    const int m = LA.dims()[0], n = LA.dims()[1];
+    if (m != n) {
+        std::fprintf(stderr, "svd: expects a square block (%d x %d)\n", m, n);
+        std::exit(1);
+    }
+    LMatrix<T> S(m, 1);
+    if (m == 0) return S;
+    std::vector<double> a((size_t)m * n), s(n);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < m; ++i) a[i + (size_t)j * m] = (double)LA(i, j);
+    DeviceBuffer<double> dA((size_t)m * n), dV((size_t)n * n), dS(n);
+    check(dh_copy_2d(handle(), DH_COPY_H2D, m, n, sizeof(double), a.data(), m, dA.get(), m), "svd");
+    int st = dh_gesvj_f64(handle(), m, n, dA.get(), m, dS.get(), dV.get(), n, nullptr);
+    if (st < 0) check(st, "svd");
+    if (st > 0) std::fprintf(stderr, "svd: Jacobi SVD did not converge\n");
+    check(dh_copy_2d(handle(), DH_COPY_D2H, n, 1, sizeof(double), dS.get(), n, s.data(), n), "svd");
+    for (int i = 0; i < m; ++i) S(i, 0) = (T)s[i];
+    return S;
+}
+
+// ---- not in the reference: the Nystrom factor of testsvm.cpp:126-172.
+// From the range finder's Q and W = K Q (n x l): Z (n x k) with
+// Z Z^T = Q V_k Lam_k V_k^T Q^T for the k largest eigenpairs of Q^T K Q
+// (where the reference takes Z = Q diag(S) from the singular values only).
+// lambda (k x 1, may be NULL) receives the eigenvalues.  Returns 0, or 1 if
+// the Jacobi iteration did not converge.
+inline int nystrom_factor(Grid g, int n, int l, int k, DMatrix<double>& Q, DMatrix<double>& W,
+                          DMatrix<double>& Z, DMatrix<double>* lambda = nullptr) {
+    (void)g;
+    int st = dh_nystrom_f64(handle(), n, l, k, Q.device_read(), Q.lld(), W.device_read(), W.lld(),
+                            Z.device_data(), Z.lld(), lambda ? lambda->device_data() : nullptr);
+    if (st < 0) check(st, "nystrom_factor");
     return st;
 }
 

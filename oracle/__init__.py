@@ -36,6 +36,8 @@ _SIG = {
     "orc_cholqr2": (_i, [_i, _i, _p, _i, _p, _i]),
     "orc_lra": (_i, [_i, _i, _i, _i, _p, _i, _p, _d, _p, _i, _i, _i, _p, _i, _p, _i]),
     "orc_lra_error": (None, [_i, _i, _i, _p, _i, _p, _d, _p, _i, _p, _i, _i, _p]),
+    "orc_gesvj": (_i, [_i, _i, _p, _i, _p, _p, _i, _d, _i]),
+    "orc_nystrom": (_i, [_i, _i, _i, _p, _i, _p, _i, _p, _i, _p]),
     "orc_set_threads": (None, [_i]),
     "orc_get_threads": (_i, []),
 }
@@ -207,3 +209,25 @@ def lra_error(X, gamma, Q, W, y=None):
     lib().orc_lra_error(n, d, l, _p_(X), d, _p_(y), gamma, _p_(Q), max(1, n), _p_(W), max(1, n),
                         0, _p_(out))
     return out[0], out[1]
+
+
+def gesvj(A, tol=1e-14, max_sweeps=60):
+    """One-sided Jacobi SVD: returns (A V = U diag(s), s descending, V, sweeps)."""
+    A = _f(A).copy(order="F")
+    m, n = A.shape
+    s = np.zeros(n)
+    V = np.zeros((n, n), order="F")
+    sweeps = lib().orc_gesvj(m, n, _p_(A), max(1, m), _p_(s), _p_(V), max(1, n), tol, max_sweeps)
+    return A, s, V, sweeps
+
+
+def nystrom(Q, W, k):
+    """Nystrom factor Z (n x k) of the range finder's Q, W = K Q and the k
+    largest eigenvalues of Q^T K Q; returns (Z, lam, status)."""
+    Q, W = _f(Q), _f(W)
+    n, l = Q.shape
+    Z = np.zeros((n, k), order="F")
+    lam = np.zeros(k)
+    st = lib().orc_nystrom(n, l, k, _p_(Q), max(1, n), _p_(W), max(1, n), _p_(Z), max(1, n),
+                           _p_(lam))
+    return Z, lam, st

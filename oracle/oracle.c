@@ -528,3 +528,144 @@ void orc_lra_error(int n, int d, int l, const double* X, int ldx, const double* 
     free(part);
     free(nrm);
 }
+
+/* ------------------------------------------------------------ Nystrom factor */
+
+/* Round-robin tournament position -> column: position 0 fixed, the others
+ * rotate by one per round.  Columns >= n are the padding player (odd n). */
+static int rr_player(int pos, int round, int N) {
+    return pos == 0 ? 0 : 1 + (pos - 1 + round) % (N - 1);
+}
+
+int orc_gesvj(int m, int n, double* A, int lda, double* sva, double* V, int ldv, double tol,
+              int max_sweeps) {
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) V[i + (size_t)j * ldv] = (i == j) ? 1.0 : 0.0;
+    const int N = n + (n & 1);
+    int sweep = 0, converged = (n < 2);
+    for (; sweep < max_sweeps && !converged; ++sweep) {
+        int rotated = 0;
+        for (int r = 0; r < N - 1; ++r) {
+            /* the pairs of one round touch disjoint columns */
+#pragma omp parallel for schedule(static) reduction(+ : rotated)
+            for (int i = 0; i < N / 2; ++i) {
+                int p = rr_player(i, r, N), q = rr_player(N - 1 - i, r, N);
+                if (p >= n || q >= n) continue;
+                if (p > q) {
+                    int t = p;
+                    p = q;
+                    q = t;
+                }
+                double* ap = A + (size_t)p * lda;
+                double* aq = A + (size_t)q * lda;
+                double al = 0.0, be = 0.0, ga = 0.0;
+                for (int e = 0; e < m; ++e) {
+                    al += ap[e] * ap[e];
+                    be += aq[e] * aq[e];
+                    ga += ap[e] * aq[e];
+                }
+                if (!(fabs(ga) > tol * sqrt(al * be))) continue;
+                double zeta = (be - al) / (2.0 * ga);
+                double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+                for (int e = 0; e < m; ++e) {
+                    double x = ap[e], y = aq[e];
+                    ap[e] = c * x - s * y;
+                    aq[e] = s * x + c * y;
+                }
+                double* vp = V + (size_t)p * ldv;
+                double* vq = V + (size_t)q * ldv;
+                for (int e = 0; e < n; ++e) {
+                    double x = vp[e], y = vq[e];
+                    vp[e] = c * x - s * y;
+                    vq[e] = s * x + c * y;
+                }
+                rotated += 1;
+            }
+        }
+        converged = (rotated == 0);
+    }
+    /* singular values = column norms, sorted descending (stable on ties) */
+    int* ord = (int*)malloc(sizeof(int) * (size_t)MAX(n, 1));
+    double* nv = (double*)malloc(sizeof(double) * (size_t)MAX(n, 1));
+    for (int j = 0; j < n; ++j) {
+        double s = 0.0;
+        for (int e = 0; e < m; ++e) s += A[e + (size_t)j * lda] * A[e + (size_t)j * lda];
+        nv[j] = sqrt(s);
+        ord[j] = j;
+    }
+    for (int a = 1; a < n; ++a) { /* insertion sort, descending */
+        int o = ord[a], b = a - 1;
+        while (b >= 0 && nv[ord[b]] < nv[o]) {
+            ord[b + 1] = ord[b];
+            --b;
+        }
+        ord[b + 1] = o;
+    }
+    double* tmpA = (double*)malloc(sizeof(double) * (size_t)MAX(m, 1) * MAX(n, 1));
+    double* tmpV = (double*)malloc(sizeof(double) * (size_t)MAX(n, 1) * MAX(n, 1));
+    for (int j = 0; j < n; ++j) {
+        memcpy(tmpA + (size_t)j * m, A + (size_t)ord[j] * lda, sizeof(double) * m);
+        memcpy(tmpV + (size_t)j * n, V + (size_t)ord[j] * ldv, sizeof(double) * n);
+        sva[j] = nv[ord[j]];
+    }
+    for (int j = 0; j < n; ++j) {
+        memcpy(A + (size_t)j * lda, tmpA + (size_t)j * m, sizeof(double) * m);
+        memcpy(V + (size_t)j * ldv, tmpV + (size_t)j * n, sizeof(double) * n);
+    }
+    free(tmpA);
+    free(tmpV);
+    free(nv);
+    free(ord);
+    return converged ? sweep : -1;
+}
+
+int orc_nystrom(int n, int l, int k, const double* Q, int ldq, const double* W, int ldw,
+                double* Z, int ldz, double* lam) {
+    double* C = (double*)malloc(sizeof(double) * (size_t)MAX(l, 1) * MAX(l, 1));
+    double* V = (double*)malloc(sizeof(double) * (size_t)MAX(l, 1) * MAX(l, 1));
+    double* sv = (double*)malloc(sizeof(double) * (size_t)MAX(l, 1));
+    double* ev = (double*)malloc(sizeof(double) * (size_t)MAX(l, 1));
+    int* ord = (int*)malloc(sizeof(int) * (size_t)MAX(l, 1));
+    /* C = Q^T W (testsvm.cpp:126-137), symmetrised */
+    orc_gemm('T', 'N', l, l, n, 1.0, Q, ldq, W, ldw, 0.0, C, l);
+    for (int j = 0; j < l; ++j)
+        for (int i = 0; i < j; ++i) {
+            double s = 0.5 * (C[i + (size_t)j * l] + C[j + (size_t)i * l]);
+            C[i + (size_t)j * l] = C[j + (size_t)i * l] = s;
+        }
+    double* C0 = (double*)malloc(sizeof(double) * (size_t)MAX(l, 1) * MAX(l, 1));
+    memcpy(C0, C, sizeof(double) * (size_t)l * l);
+    int st = orc_gesvj(l, l, C, l, sv, V, l, 1e-14, 60);
+    /* signed eigenvalues lam_i = v_i . (C0 v_i); C holds C0 V after the sweeps */
+    for (int j = 0; j < l; ++j) {
+        double s = 0.0;
+        for (int e = 0; e < l; ++e) s += V[e + (size_t)j * l] * C[e + (size_t)j * l];
+        ev[j] = s;
+        ord[j] = j;
+    }
+    for (int a = 1; a < l; ++a) {
+        int o = ord[a], b = a - 1;
+        while (b >= 0 && ev[ord[b]] < ev[o]) {
+            ord[b + 1] = ord[b];
+            --b;
+        }
+        ord[b + 1] = o;
+    }
+    /* Vk = V[:, top k] diag(sqrt(max(lam, 0))); Z = Q Vk */
+    double* Vk = (double*)malloc(sizeof(double) * (size_t)MAX(l, 1) * MAX(k, 1));
+    for (int j = 0; j < k; ++j) {
+        double lj = ev[ord[j]], sc = lj > 0.0 ? sqrt(lj) : 0.0;
+        if (lam) lam[j] = lj;
+        for (int e = 0; e < l; ++e) Vk[e + (size_t)j * l] = V[e + (size_t)ord[j] * l] * sc;
+    }
+    orc_gemm('N', 'N', n, k, l, 1.0, Q, ldq, Vk, l, 0.0, Z, ldz);
+    free(Vk);
+    free(C0);
+    free(ord);
+    free(ev);
+    free(sv);
+    free(V);
+    free(C);
+    return st < 0 ? 1 : 0;
+}

@@ -130,6 +130,35 @@ int orc_lra(int n, int d, int l, int q, const double* X, int ldx, const double* 
 void orc_lra_error(int n, int d, int l, const double* X, int ldx, const double* y, double gamma,
                    const double* Q, int ldq, const double* W, int ldw, int chunk, double* err2);
 
+/* ---- the Nystrom factor of the range finder (testsvm.cpp:126-177) ---- */
+
+/* One-sided (Hestenes) Jacobi SVD of A (m x n, m >= n): columns are
+ * orthogonalised pairwise, pairs visited in the round-robin (tournament)
+ * order -- round r pairs positions i and N-1-i, N = n rounded up to even,
+ * position 0 fixed and player(p) = 1 + (p - 1 + r) mod (N - 1) -- until a
+ * sweep rotates no pair, a pair (p, q) rotating when
+ * |a_p.a_q| > tol sqrt(|a_p|^2 |a_q|^2).  Rotation from the Rutishauser
+ * formulas (Golub & Van Loan, Matrix Computations, 4th ed., Alg. 8.5.2 /
+ * 8.6.3): zeta = (|a_q|^2 - |a_p|^2) / (2 a_p.a_q),
+ * t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)), c = 1/sqrt(1+t^2), s = c t,
+ * a_p <- c a_p - s a_q, a_q <- s a_p + c a_q (and likewise V).  On exit A =
+ * U diag(sva) (columns unnormalised), V (n x n) the right singular vectors,
+ * both permuted so sva is descending.  Replaces sgesvd_ in DArray::svd
+ * (lapack_like.h:388-413).  Returns the sweeps used, or -1 if max_sweeps
+ * did not converge. */
+int orc_gesvj(int m, int n, double* A, int lda, double* sva, double* V, int ldv, double tol,
+              int max_sweeps);
+
+/* Nystrom factor of the range finder's output: C = (Q^T W + W^T Q) / 2 (l x l,
+ * W = K Q, so C = Q^T K Q as testsvm.cpp:126-137 forms it), C = V diag(lam) V^T
+ * by orc_gesvj (lam_i = v_i . (C v_i), signed), the k largest lam in
+ * descending order, Z = Q V_k diag(sqrt(max(lam_k, 0))) (n x k), so Z Z^T =
+ * Q V_k Lam_k V_k^T Q^T.  The reference instead takes Z = Q diag(S) from
+ * the singular values alone (testsvm.cpp:146-172).  lam (k) may be NULL.
+ * Returns the gesvj status. */
+int orc_nystrom(int n, int l, int k, const double* Q, int ldq, const double* W, int ldw,
+                double* Z, int ldz, double* lam);
+
 /* Threads used by the OpenMP-parallel loops (1 = the sequential-BLAS link DArray uses). */
 void orc_set_threads(int nthreads);
 int orc_get_threads(void);

@@ -45,6 +45,15 @@ double rel_err(DArray::DMatrix<T>& A, const std::vector<double>& want, int n, in
     return std::sqrt(e / s);
 }
 
+const char* prec_name(DArray::Precision p) {
+    switch (p) {
+        case DArray::Precision::FP64: return "fp64";
+        case DArray::Precision::Mixed: return "mixed (fp16)";
+        case DArray::Precision::BF16: return "bf16";
+        default: return "fp8";
+    }
+}
+
 template <typename T>
 int run_lra(int n, int d, int k, DArray::Precision prec, double tol) {
     DArray::Grid g;
@@ -64,7 +73,7 @@ int run_lra(int n, int d, int k, DArray::Precision prec, double tol) {
     DArray::LRA(g, n, d, k, X, Y, O, A, (float)gamma, prec);
     double e = rel_err(A, want, n, k);
     std::printf("LRA<%s> %s n=%d d=%d k=%d: rel Frobenius error vs host %.3e (tol %.0e)\n",
-                sizeof(T) == 4 ? "float" : "double", prec == DArray::Precision::FP64 ? "fp64" : "mixed",
+                sizeof(T) == 4 ? "float" : "double", prec_name(prec),
                 n, d, k, e, tol);
     int bad = e > tol;
     // Knorm with Z = 0: ||K - 0|| = ||K||, ratio exactly 1.
@@ -91,7 +100,7 @@ int run_range_finder() {
         double e = DArray::lra_error(g, n, d, l, X, nullptr, Q, W, gamma);
         double tol = prec == DArray::Precision::FP64 ? 1e-10 : 1e-2;
         std::printf("range finder %s, rank-%d kernel, l=%d: status %d, ||K-QB||/||K|| = %.3e (tol %.0e)\n",
-                    prec == DArray::Precision::FP64 ? "fp64" : "mixed", r, l, st, e, tol);
+                    prec_name(prec), r, l, st, e, tol);
         bad += (e > tol);
     }
     return bad;
@@ -101,6 +110,7 @@ int main() {
     int bad = 0;
     bad += run_lra<double>(1024, 16, 40, DArray::Precision::FP64, 1e-12);
     bad += run_lra<double>(1024, 16, 40, DArray::Precision::Mixed, 2e-2);
+    bad += run_lra<double>(1024, 16, 40, DArray::Precision::BF16, 2e-2);
     bad += run_lra<float>(1024, 16, 40, DArray::Precision::FP64, 1e-6);
     bad += run_range_finder();
     std::printf("%s\n", bad ? "FAIL" : "PASS");

@@ -78,3 +78,18 @@ def test_views_multiply_and_solve():
 @pytest.mark.gpu
 def test_lra_knorm_and_range_finder():
     _run(["lra_knorm"], timeout=300)
+
+
+@pytest.mark.gpu
+def test_reference_seam_rbf_kernel_tc_gemm_tc_sgemm():
+    """rbf_kernel / tc_gemm / tc_sgemm through their C++ linkage with host
+    layouts, padded leading dimensions, beta, inner dimension > 8192 and all
+    four transposes, against host sums in double."""
+    _run(["seam"], timeout=300)
+
+
+@pytest.mark.gpu
+def test_svd_and_nystrom_factor_with_knorm():
+    """DArray::svd known answers and the range finder -> Nystrom Z -> Knorm flow
+    of testsvm.cpp:86-177 through the header mirror."""
+    _run(["nystrom_knorm"], timeout=300)

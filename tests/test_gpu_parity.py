@@ -468,3 +468,75 @@ def test_posv_mixed_falls_back_to_fp64(dev):
     info, iters, berr = dev.posv_mixed(dev_mat(A), dev_mat(B), X, 64, 1e-10, 30)
     assert info == 0 and iters == -1
     assert rel(A @ dh.from_dev(X), B) < 1e-5
+
+
+@pytest.mark.parametrize("fn", ["potrf", "posv"])
+def test_lookahead_cholesky_with_split_k_updates(dev, fn):
+    """bs = 1024 at n = 4096: the trailing updates have k = 1024 and few output
+    tiles, so they split k on both the handle's stream and the look-ahead
+    stream, which run concurrently with separate partials buffers."""
+    import os
+    n, bs = 4096, 1024
+    rng = np.random.default_rng(31)
+    M = rng.standard_normal((n, n))
+    A = M.T @ M / n + np.eye(n)
+    orc.set_threads(min(16, os.cpu_count() or 1))
+    try:
+        U, info = orc.potrf_upper(A, bs)
+    finally:
+        orc.set_threads(1)
+    assert info == 0
+    Ad = dev_mat(A)
+    if fn == "potrf":
+        assert dev.potrf(Ad, bs) == 0
+    else:
+        B = rng.standard_normal((n, 8))
+        Bd = dev_mat(B)
+        assert dev.posv(Ad, Bd, bs) == 0
+        assert rel(A @ dh.from_dev(Bd), B) < 1e-10
+    e = rel(np.triu(dh.from_dev(Ad)), np.triu(U))
+    print(f"{fn} n={n} bs={bs}: factor vs oracle {e:.2e}", flush=True)
+    assert e < 1e-12
+
+
+def test_posv_mixed_max_iter_reached_falls_back(dev):
+    """A tolerance the FP32 refinement cannot reach within one correction: the
+    solve is redone in FP64 (iters = -1) and the returned X is the FP64 one."""
+    import torch
+    n = 400
+    A = _rbf_core(n, 16)
+    B = np.random.default_rng(24).standard_normal((n, 4))
+    X = torch.zeros((4, n), dtype=torch.float64, device="cuda")
+    info, iters, berr = dev.posv_mixed(dev_mat(np.triu(A)), dev_mat(B), X, 64, 1e-15, 1)
+    assert info == 0 and iters == -1
+    assert berr <= 1e3 * n * 2.0**-53
+    assert rel(dh.from_dev(X), np.linalg.solve(A, B)) < 1e-6
+
+
+def test_posv_mixed_diverging_refinement_falls_back(dev):
+    """kappa = 1e9: u32 * kappa > 1, so FP32 refinement cannot converge (or the
+    FP32 factorisation breaks down); either way the FP64 path answers."""
+    import torch
+    rng = np.random.default_rng(25)
+    n = 300
+    Qm, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = (Qm * np.logspace(0, -9, n)) @ Qm.T
+    A = (A + A.T) / 2
+    B = rng.standard_normal((n, 2))
+    X = torch.zeros((2, n), dtype=torch.float64, device="cuda")
+    info, iters, berr = dev.posv_mixed(dev_mat(A), dev_mat(B), X, 64, 1e-10, 30)
+    print(f"kappa 1e9: info {info} iters {iters} berr {berr:.2e}", flush=True)
+    assert info == 0 and iters == -1 and berr <= 1e-10
+
+
+def test_posv_mixed_nan_rhs_is_reported(dev):
+    """A NaN in B gives a non-finite backward error: never reported as converged."""
+    import torch
+    n = 128
+    A = _rbf_core(n, 8)
+    B = np.ones((n, 2))
+    B[5, 1] = np.nan
+    X = torch.zeros((2, n), dtype=torch.float64, device="cuda")
+    with pytest.raises(dh.DHError) as ei:
+        dev.posv_mixed(dev_mat(np.triu(A)), dev_mat(B), X, 64, 1e-10, 10)
+    assert ei.value.status == -1003

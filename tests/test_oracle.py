@@ -251,3 +251,77 @@ def test_lra_householder_and_cholqr2_same_subspace():
     Q2, W2, _ = orc.lra(X, Om, 0.4, 2, orth="cholqr2")
     assert rel(Q1 @ Q1.T, Q2 @ Q2.T) < 1e-9
     assert rel(Q1 @ W1.T, Q2 @ W2.T) < 1e-9
+
+
+# ---------------------------------------------------------------- Nystrom factor
+
+@pytest.mark.parametrize("m,n", [(50, 50), (300, 272), (273, 273), (100, 3), (7, 1)])
+def test_gesvj_vs_numpy_svd(m, n):
+    rng = np.random.default_rng(m * 1000 + n)
+    A = rng.standard_normal((m, n))
+    AV, s, V, sweeps = orc.gesvj(A)
+    assert sweeps > 0 or n == 1
+    ref = np.linalg.svd(A, compute_uv=False)
+    assert np.abs(s - ref).max() <= 1e-13 * ref[0]
+    assert np.all(np.diff(s) <= 0)
+    assert np.linalg.norm(V.T @ V - np.eye(n)) < 1e-12
+    # A V = U diag(s) with orthonormal U: the columns of A V are orthogonal, of norm s
+    G = AV.T @ AV
+    assert np.abs(G - np.diag(s ** 2)).max() <= 1e-12 * s[0] ** 2
+    assert rel(AV, A @ V) < 1e-13
+
+
+def test_gesvj_closed_forms():
+    # A diagonal matrix needs no rotation: exact |d| in descending order, V a permutation.
+    d = np.array([3.0, -7.0, 0.5, 0.0, 2.0])
+    AV, s, V, sweeps = orc.gesvj(np.diag(d))
+    assert sweeps == 1 and list(s) == [7.0, 3.0, 2.0, 0.5, 0.0]
+    assert np.array_equal(np.abs(V), np.abs(V).round())
+    # [[a, b], [b, a]]: eigenvalues a + b, a - b with vectors (1, 1)/sqrt2, (1, -1)/sqrt2.
+    AV, s, V, _ = orc.gesvj(np.array([[2.0, 1.0], [1.0, 2.0]]))
+    np.testing.assert_allclose(s, [3.0, 1.0], rtol=1e-15)
+    np.testing.assert_allclose(np.abs(V), np.full((2, 2), np.sqrt(0.5)), rtol=1e-15)
+
+
+def _nystrom_numpy(Q, W, k):
+    C = Q.T @ W
+    C = (C + C.T) / 2
+    w, U = np.linalg.eigh(C)
+    idx = np.argsort(w)[::-1][:k]
+    return Q @ (U[:, idx] * np.sqrt(np.maximum(w[idx], 0))), w[idx]
+
+
+@pytest.mark.parametrize("labels", [False, True])
+def test_nystrom_vs_numpy_eigh(labels):
+    rng = np.random.default_rng(40)
+    n, d, l, k = 600, 8, 50, 40
+    X = rng.uniform(-1, 1, (n, d))
+    y = rng.choice([-1.0, 1.0], n) if labels else None
+    Om = rng.standard_normal((n, l))
+    Q, W, info = orc.lra(X, Om, 0.5, 1, y)
+    assert info == 0
+    Z, lam, st = orc.nystrom(Q, W, k)
+    assert st == 0
+    Zr, lr = _nystrom_numpy(Q, W, k)
+    assert np.abs(lam - lr).max() <= 1e-12 * lr[0]
+    assert rel(Z @ Z.T, Zr @ Zr.T) < 1e-12
+    # K - Z Z^T against the rank-k truncation of Q Q^T K Q Q^T: no better than the range finder
+    e2, k2 = orc.lra_error(X, 0.5, Z, Z, y)
+    q2, _ = orc.lra_error(X, 0.5, Q, W, y)
+    assert q2 <= e2 * (1 + 1e-12)
+
+
+def test_nystrom_exact_for_low_rank_kernel():
+    # r distinct points: rank(K) <= r <= k, so Z Z^T reproduces K to rounding.
+    rng = np.random.default_rng(41)
+    r, n, d = 12, 900, 5
+    P = rng.uniform(-1, 1, (r, d))
+    X = P[np.arange(n) % r]
+    Om = rng.standard_normal((n, 20))
+    # l = 20 > rank: Householder orthogonalisation (CholeskyQR breaks down here)
+    Q, W, info = orc.lra(X, Om, 0.7, 1, orth="householder")
+    assert info == 0
+    Z, lam, st = orc.nystrom(Q, W, 16)
+    e2, k2 = orc.lra_error(X, 0.7, Z, Z)
+    assert st == 0 and np.sqrt(e2 / k2) < 1e-12
+    assert np.all(lam[r:] < 1e-10 * lam[0])  # eigenvalues past the rank vanish
