@@ -29,6 +29,7 @@ sys.path.insert(0, str(ROOT))
 
 PEAK_FP64_MFMA = 78.6e12  # MI355X dense FP64 matrix, FLOP/s
 PEAK_BF16_MFMA = 2.5e15  # MI355X dense BF16 matrix, FLOP/s
+PEAK_FP8_MFMA = 5.0e15  # MI355X dense FP8 matrix, FLOP/s
 
 
 def measured_traffic(path):
@@ -49,21 +50,70 @@ def lra_flops(n, d, l, q):
     return f_mfma, f_orth
 
 
-def cpu_baseline(X, Om, gamma, rows, threads):
-    """Oracle (FP64 CPU restatement) on a bounded sample of the same workload:
-    rows [0, rows) of one K*Omega pass, i.e. 2*rows*n*(d+l) flops."""
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads() -> int:
+    """Threads for the all-cores CPU mode: the CPUs this process may run on,
+    at most 16 (the host CPU share of one GPU on the measurement boxes)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
+def cpu_baseline(X1, Om1, gamma1, rows):
+    """The FP64 CPU restatement of DArray's path (the oracle), timed on this
+    host in the same run: (1) the full C0 LRA -- n = 8192 points uniform in
+    [0,1)^64 (set_uniform seed 0), Omega from set_normal_seed(1), l = 138,
+    q = 1, Householder orthogonalisation as DArray's TSQR -- with 1 thread
+    (DArray's one-rank sequential-BLAS link) and with all host threads;
+    (2) a bounded sample of the C1 workload: rows [0, rows) of one K*Omega
+    pass with all host threads."""
+    import darray_optimizer_amd as dh
     import oracle as orc
+    threads = host_threads()
+    n0, d0, l0, q0 = 8192, 64, 138, 1
+    X0 = np.ascontiguousarray(dh.fill_uniform(0, d0, n0).T)
+    Om0 = np.asfortranarray(dh.fill_normal(1, n0, l0))
+    fm0, fo0 = lra_flops(n0, d0, l0, q0)
+    modes = {}
+    for name, th in (("single_thread", 1), ("all_threads", threads)):
+        orc.set_threads(th)
+        t0 = time.perf_counter()
+        Q0, W0, info = orc.lra(X0, Om0, 1.0 / d0, q0, orth="householder")
+        dt = time.perf_counter() - t0
+        modes[name] = {"threads": th, "s_per_solve": dt, "gflops": (fm0 + fo0) / dt / 1e9,
+                       "info": info}
+    e2, k2 = orc.lra_error(X0, 1.0 / d0, Q0, W0)
     orc.set_threads(threads)
-    n, d = X.shape
-    l = Om.shape[1]
-    Xi = np.ascontiguousarray(X[:rows])
+    n, d = X1.shape
+    l = Om1.shape[1]
+    Xi = np.ascontiguousarray(X1[:rows])
     t0 = time.perf_counter()
-    orc.rbf_project(Xi, X, Om, gamma)
+    orc.rbf_project(Xi, X1, Om1, gamma1)
     dt = time.perf_counter() - t0
-    flops = 2.0 * rows * n * (d + l)
-    return {"value": flops / dt / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": "port",
-            "sample": f"rows 0..{rows} of one K*Omega pass (m={rows}, n={n}, d={d}, l={l}), "
-                      f"FP64 oracle, {dt:.2f} s"}
+    orc.set_threads(1)
+    allc = modes["all_threads"]
+    return {"value": allc["gflops"], "unit": "GFLOP/s", "cores": threads, "kind": "port",
+            "sample": f"full FP64 LRA at n={n0}, d={d0}, l={l0}, q={q0} (uniform points, "
+                      f"Householder orthogonalisation), oracle on {threads} threads, "
+                      f"{allc['s_per_solve']:.2f} s",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "c0_rel_err": float(np.sqrt(max(e2, 0.0) / k2)),
+            "single_thread": modes["single_thread"], "all_threads": allc,
+            "c1_kpass_sample": {"rows": rows, "threads": threads, "s": dt,
+                                "gflops": 2.0 * rows * n * (d + l) / dt / 1e9,
+                                "sample": f"rows 0..{rows} of one K*Omega pass (n={n}, d={d}, "
+                                          f"l={l})"}}
 
 
 def core_solve(h, Xd, gamma, nc, nrhs, stream):
@@ -141,7 +191,14 @@ def batched_workload(h, count, seed, with_ref):
         h.lra_batched(probs, prec=dh.DH_PREC_BF16)
     torch.cuda.synchronize()
     s = (time.perf_counter() - t0) / reps
+    f_pass = sum(lra_flops(p["X"].shape[0], p["X"].shape[1], p["Omega"].shape[0], 1)[0]
+                 for p in probs)
     out = {"problems": count, "p": 10, "q": 1, "s_per_batch": s, "gflops": flops / s / 1e9,
+           "roofline": {"bound": "mfma", "achieved": f_pass / s / 1e12,
+                        "peak": PEAK_BF16_MFMA / 1e12, "unit": "TFLOP/s",
+                        "frac": f_pass / s / PEAK_BF16_MFMA,
+                        "kernel": "whole batch (persistent K-pass kernels + preparation + "
+                                  "CholeskyQR2), K-pass flops only"},
            "n_range": [min(p["X"].shape[0] for p in probs), max(p["X"].shape[0] for p in probs)]}
     if with_ref:
         # Error ratio vs FP64 on every 8th problem (FP64 solve + FP64 error pass each).
@@ -157,6 +214,54 @@ def batched_workload(h, count, seed, with_ref):
     del probs
     torch.cuda.empty_cache()
     return out
+
+
+def consumer_steps(h, Xd, X, gamma, n, d, l, k, q):
+    """The steps after the range finder (testsvm.cpp:126-321) at the headline
+    size: labelled FP16 range finder (labels +-1 by nearest-centre parity of the
+    same mixture), Nystrom factor Z (k columns) with ||K - Z Z^T||, and the
+    barrier SVM solver (C = 1) on Z."""
+    import torch
+    import darray_optimizer_amd as dh
+    _, y = dh.fill_mixture(0, d, n, 16, 0.25, labels=True)
+    yd = torch.from_numpy(y).cuda()
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(7)
+    Om = torch.randn((l, n), dtype=torch.float64, device="cuda", generator=gen)
+    Q = torch.empty((l, n), dtype=torch.float64, device="cuda")
+    W = torch.empty_like(Q)
+    assert h.lra(Xd, Om, gamma, q, Q, W, yd, prec=dh.DH_PREC_F16) == 0
+    Z = torch.zeros((k, n), dtype=torch.float64, device="cuda")
+    lam = torch.zeros(k, dtype=torch.float64, device="cuda")
+    assert h.nystrom(Q, W, k, Z, lam) == 0  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    assert h.nystrom(Q, W, k, Z, lam) == 0
+    torch.cuda.synchronize()
+    t_ny = time.perf_counter() - t0
+    e2, k2 = h.lra_error(Xd, gamma, Z, Z, yd)
+    q2, _ = h.lra_error(Xd, gamma, Q, W, yd)
+    a = torch.zeros(n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats, f = h.svm_barrier(Z, yd, 1.0, a)
+    torch.cuda.synchronize()
+    t_svm = time.perf_counter() - t0
+    ah, Zh = a.cpu().numpy(), Z.cpu().numpy().T
+    s = Zh @ (Zh.T @ ah)
+    free = np.flatnonzero((ah > 1e-6) & (ah < 1.0 - 1e-6))[:50]
+    b = float(np.mean(y[free] * (1 - s[free]))) if len(free) else 0.0
+    acc = float(np.mean(np.sign(y * s + b) == y))
+    del Q, W, Om
+    torch.cuda.empty_cache()
+    return {"labels": "+-1 by nearest-centre parity", "nystrom": {
+                "k": k, "ms": t_ny * 1e3, "rel_err_ZZt": float(np.sqrt(max(e2, 0.0) / k2)),
+                "rel_err_QB": float(np.sqrt(max(q2, 0.0) / k2)),
+                "lambda_1": float(lam[0]), "lambda_k": float(lam[-1])},
+            "svm_barrier": {"C": 1.0, "s": t_svm, "barrier_iterations": stats[0],
+                            "newton_steps": stats[1], "cg_iterations": stats[2], "f": f,
+                            "training_accuracy": acc,
+                            "z_bytes_per_cg_iteration": 2 * 8 * n * k}}
 
 
 def largest_workload(h, n, d, k, p, q, with_ref):
@@ -180,10 +285,26 @@ def largest_workload(h, n, d, k, p, q, with_ref):
     assert h.lra(X, Om, gamma, q, Q, W, prec=dh.DH_PREC_FP8) == 0
     torch.cuda.synchronize()
     s = time.perf_counter() - t0
+    # Kernel time of the fused FP8-distance passes (HIP events on their stream).
+    h.timing(True)
+    h.timing_read()
+    assert h.lra(X, Om, gamma, q, Q, W, prec=dh.DH_PREC_FP8) == 0
+    kms, launches = h.timing_read()
+    h.timing(False)
+    passes = q + 2
+    f_dist, f_proj = 2.0 * n * n * d, 2.0 * n * n * l
+    t_pass = kms / 1e3 / passes
+    t_min = f_dist / PEAK_FP8_MFMA + f_proj / PEAK_BF16_MFMA
+    roof = {"bound": "mfma", "achieved": (f_dist + f_proj) / t_pass / 1e12,
+            "peak": (f_dist + f_proj) / t_min / 1e12, "unit": "TFLOP/s", "frac": t_min / t_pass,
+            "kernel": "rbf_project_bf16_kernel<8,17,4,false,2,D8> (FP8 distance, BF16 projection)",
+            "kernel_ms_per_pass": t_pass * 1e3, "launches_per_pass": launches / passes,
+            "peak_note": "blended: 2n^2 d at the 5.0 PFLOP/s FP8 peak + 2n^2 l at the "
+                         "2.5 PFLOP/s BF16 peak"}
     out = {"n": n, "d": d, "k": k, "p": p, "q": q, "l": l, "gamma": gamma, "centres": 32,
            "sigma": 0.2, "precision": "fp8 e4m3 distance (per-row scales), bf16 projection, "
                                       "fp64 CholeskyQR2", "s_per_solve": s,
-           "gflops": (fm + fo) / s / 1e9}
+           "gflops": (fm + fo) / s / 1e9, "roofline": roof}
     if with_ref:
         e2, k2 = h.lra_error(X, gamma, Q, W)
         out["rel_err"] = float(np.sqrt(max(e2, 0.0) / k2))
@@ -218,12 +339,14 @@ def main(argv=None) -> int:
     ap.add_argument("--k", type=int, default=256)
     ap.add_argument("--p", type=int, default=16)
     ap.add_argument("--q", type=int, default=2)
-    ap.add_argument("--cpu-rows", type=int, default=65536)
+    ap.add_argument("--cpu-rows", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--core", type=int, default=16384, help="core order (0: skip)")
     ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--batched", type=int, default=96, help="batched problems (0: skip)")
+    ap.add_argument("--no-consumer", dest="consumer", action="store_false",
+                    help="skip the Nystrom factor and the barrier SVM solver")
     ap.add_argument("--largest", type=int, default=262144,
                     help="points of the FP8-distance workload (0: skip)")
     a = ap.parse_args(argv)
@@ -329,6 +452,8 @@ def main(argv=None) -> int:
         line["err_ratio_vs_fp64"] = mixed["rel_err"] / fp64["rel_err"]
         bf16["err_ratio_vs_fp64"] = bf16["rel_err"] / fp64["rel_err"]
     bf16["speedup_vs_fp64"] = fp64["s_per_solve"] / bf16["s_per_solve"]
+    if a.consumer:
+        line["consumer"] = consumer_steps(h, Xd, X, gamma, n, d, l, a.k, q)
     if a.batched > 0:
         line["batched"] = batched_workload(h, a.batched, 3, not a.no_error)
     if a.largest > 0:
@@ -336,8 +461,7 @@ def main(argv=None) -> int:
     if a.core > 0:
         line["core_solve"] = core_solve(h, Xd, gamma, a.core, a.nrhs, stream)
     if not a.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline(X, Om, gamma, a.cpu_rows, threads)
+        line["cpu_baseline"] = cpu_baseline(X, Om, gamma, a.cpu_rows)
     print(json.dumps(line), flush=True)
     h.close()
     return 0

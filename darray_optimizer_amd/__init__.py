@@ -233,6 +233,26 @@ class Handle:
         return call("dh_nystrom_f64", self.h, n, l, k, _ptr(Q), _ld(Q), _ptr(W), _ld(W), _ptr(Z),
                     _ld(Z), _ptr(lam))
 
+    def svm_pcg(self, Z, D, b, x, tol=1e-2, max_iter=1000):
+        """(Z Z^T + diag(D)) x = b by preconditioned CG; Z a (k, n) tensor (n x k
+        column-major), x the initial guess (updated).  Returns (iters, rnorm)."""
+        n, k = Z.shape[1], Z.shape[0]
+        it, rn = C.c_int(0), C.c_double(0.0)
+        call("dh_svm_pcg", self.h, n, k, _ptr(Z), _ld(Z), _ptr(D), _ptr(b), _ptr(x), float(tol),
+             int(max_iter), C.byref(it), C.byref(rn))
+        return it.value, rn.value
+
+    def svm_barrier(self, Z, y, C_, a, cg_tol=1e-2, max_outer=50, max_newton=100, max_cg=None):
+        """Barrier-method SVM dual on the Nystrom factor Z ((k, n) tensor); a (n)
+        receives the solution.  Returns (stats [outer, newton, cg], f(a))."""
+        n, k = Z.shape[1], Z.shape[0]
+        st = (C.c_int * 3)()
+        fv = C.c_double(0.0)
+        call("dh_svm_barrier", self.h, n, k, _ptr(Z), _ld(Z), _ptr(y), float(C_), float(cg_tol),
+             int(max_outer), int(max_newton), int(max_cg if max_cg is not None else 10 * n),
+             _ptr(a), st, C.byref(fv))
+        return list(st), fv.value
+
     def lra_error(self, X, gamma, Q, W, y=None):
         """(||K - Q W^T||_F^2, ||K||_F^2) in FP64."""
         n, d = X.shape

@@ -63,6 +63,9 @@ SIGNATURES: dict[str, tuple] = {
     "dh_lra_error": (_i, [_p, _i, _i, _i, _p, _i, _p, _d, _p, _i, _p, _i, C.POINTER(_d)]),
     "dh_gesvj_f64": (_i, [_p, _i, _i, _p, _i, _p, _p, _i, C.POINTER(_i)]),
     "dh_nystrom_f64": (_i, [_p, _i, _i, _i, _p, _i, _p, _i, _p, _i, _p]),
+    "dh_svm_pcg": (_i, [_p, _i, _i, _p, _i, _p, _p, _p, _d, _i, C.POINTER(_i), C.POINTER(_d)]),
+    "dh_svm_barrier": (_i, [_p, _i, _i, _p, _i, _p, _d, _d, _i, _i, _i, _p, C.POINTER(_i),
+                            C.POINTER(_d)]),
     "dh_read_libsvm": (_i, [C.c_char_p, _i, _i, C.POINTER(_d), _i, C.POINTER(_d)]),
     "dh_write_libsvm_model": (_i, [C.c_char_p, _i, _i, _i, C.POINTER(_d), _d, C.POINTER(_d), _i,
                                    C.POINTER(_d), C.POINTER(_d), _i, _d, C.POINTER(_d)]),

@@ -28,6 +28,7 @@ enum dh_slot {
     DH_WS_TRSM_INV,    // inverses of the 64 x 64 diagonal blocks of a triangular solve
     DH_WS_TRSM_X,      // solution blocks of the block-sweep triangular solve
     DH_WS_EIG,         // Jacobi SVD / Nystrom work matrices
+    DH_WS_SVM,         // vectors and partial sums of the barrier SVM solver
     DH_WS_COUNT
 };
 

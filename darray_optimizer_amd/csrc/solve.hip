@@ -184,9 +184,11 @@ extern "C" int dh_posv_mixed(dh_handle_t h, char uplo, int n, int nrhs, const do
         const double anorm = std::sqrt(anorm2);
         double be = 0.0;
         for (int j = 0; j < nrhs; ++j) {
-            double den = anorm * std::sqrt(hn[nrhs + j]) + std::sqrt(hn[j]);
-            double e = den > 0.0 ? std::sqrt(hn[2 * nrhs + j]) / den : 0.0;
-            if (!std::isfinite(e)) e = INFINITY;  // NaN must not compare as converged
+            const double den = anorm * std::sqrt(hn[nrhs + j]) + std::sqrt(hn[j]);
+            const double rn = std::sqrt(hn[2 * nrhs + j]);
+            // A NaN anywhere (residual, X or B) must not compare as converged.
+            double e = den > 0.0 ? rn / den : (rn == 0.0 ? 0.0 : INFINITY);
+            if (!std::isfinite(e)) e = INFINITY;
             be = std::max(be, e);
         }
         *out = be;

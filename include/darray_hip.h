@@ -295,6 +295,36 @@ int dh_gesvj_f64(dh_handle_t h, int m, int n, double* A, int lda, double* sva, d
 int dh_nystrom_f64(dh_handle_t h, int n, int l, int k, const double* Q, int ldq, const double* W,
                    int ldw, double* Z, int ldz, double* lambda);
 
+/* ------------------------------------------------- barrier SVM solver */
+
+/* Preconditioned conjugate gradients for (Z Z^T + diag(D)) x = b, Z n x k
+ * (ld ldz, k <= 2048), D, b, x (n; x holds the initial guess on entry and
+ * the solution on exit), preconditioner diag(Z Z^T) + D, until
+ * ||b - (Z Z^T + D) x||_2 <= tol or max_iter iterations: pre_conjgrad
+ * (dbarrier.h:83-139), whose operator Z Z^T + D + n I is this one with D + n.
+ * *iters and *rnorm (host, may be NULL) receive the iterations and the final
+ * residual norm.  Synchronises. */
+int dh_svm_pcg(dh_handle_t h, int n, int k, const double* Z, int ldz, const double* D,
+               const double* b, double* x, double tol, int max_iter, int* iters, double* rnorm);
+
+/* Barrier-method solver of the SVM dual on the Nystrom factor Z (n x k) of
+ * diag(y) K diag(y), as testsvm.cpp:191-321 runs it: min a^T Z Z^T a / 2 -
+ * sum(a) over 0 < a < C (no equality constraint), starting from the smaller
+ * class at 0.9 C and the other scaled so that y^T a = 0; Newton steps on
+ * g = f - sum(log a + log(C - a)) / t with the step from dh_svm_pcg's CG
+ * ((Z Z^T + diag(D)) s = -grad, from s = 0, ||r|| <= cg_tol, max_cg
+ * iterations), backtracking line search (0.1, 0.5), Newton stops at
+ * decrement^2/2 < 1e-6, slow progress or max D / min D > 1e18, and
+ * t *= 20 until 2n/t < 1e-5 |f|.  The reference solves its Newton system
+ * with the right-hand side e_1 and an added n I (testsvm.cpp:234-235,
+ * dbarrier.h:99); here it is the Newton system.  y (n, DEVICE, +-1) and a
+ * (n, DEVICE, output).  stats (host, may be NULL) = barrier iterations,
+ * Newton steps, CG iterations; *fval (host, may be NULL) = f(a).  Returns
+ * -6 if a class is empty.  Synchronises. */
+int dh_svm_barrier(dh_handle_t h, int n, int k, const double* Z, int ldz, const double* y,
+                   double C, double cg_tol, int max_outer, int max_newton, int max_cg, double* a,
+                   int* stats, double* fval);
+
 /* ------------------------------------------------------ LIBSVM files */
 /* Host functions (HOST pointers; no GPU use). */
 

@@ -38,6 +38,7 @@ _SIG = {
     "orc_lra_error": (None, [_i, _i, _i, _p, _i, _p, _d, _p, _i, _p, _i, _i, _p]),
     "orc_gesvj": (_i, [_i, _i, _p, _i, _p, _p, _i, _d, _i]),
     "orc_nystrom": (_i, [_i, _i, _i, _p, _i, _p, _i, _p, _i, _p]),
+    "orc_svm_barrier": (_i, [_i, _i, _p, _i, _p, _d, _d, _i, _i, _i, _p, _p, _p]),
     "orc_set_threads": (None, [_i]),
     "orc_get_threads": (_i, []),
 }
@@ -231,3 +232,19 @@ def nystrom(Q, W, k):
     st = lib().orc_nystrom(n, l, k, _p_(Q), max(1, n), _p_(W), max(1, n), _p_(Z), max(1, n),
                            _p_(lam))
     return Z, lam, st
+
+
+def svm_barrier(Z, y, Cb, cg_tol=1e-2, max_outer=50, max_newton=100, max_cg=None):
+    """Barrier-method SVM dual on the Nystrom factor Z (n x k); returns
+    (a, [outer, newton, cg], f(a), status)."""
+    Z = _f(Z)
+    n, k = Z.shape
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    a = np.zeros(n)
+    st = np.zeros(3, dtype=np.int32)
+    fv = C.c_double(0.0)
+    r = lib().orc_svm_barrier(n, k, _p_(Z), max(1, n), _p_(y), float(Cb), float(cg_tol),
+                              int(max_outer), int(max_newton),
+                              int(max_cg if max_cg is not None else 10 * n), _p_(a), _p_(st),
+                              C.byref(fv))
+    return a, [int(v) for v in st], fv.value, r

@@ -669,3 +669,159 @@ int orc_nystrom(int n, int l, int k, const double* Q, int ldq, const double* W, 
     free(C);
     return st < 0 ? 1 : 0;
 }
+
+/* ------------------------------------------------------ barrier SVM solver */
+
+/* u = Z^T x (k) */
+static void zt_x(int n, int k, const double* Z, int ldz, const double* x, double* u) {
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < k; ++c) {
+        const double* z = Z + (size_t)c * ldz;
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += z[i] * x[i];
+        u[c] = s;
+    }
+}
+
+/* out = Z u (n) */
+static void z_u(int n, int k, const double* Z, int ldz, const double* u, double* out) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int c = 0; c < k; ++c) s += Z[i + (size_t)c * ldz] * u[c];
+        out[i] = s;
+    }
+}
+
+static double dot_n(int n, const double* a, const double* b) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+
+/* sum(log(a + tt s) + log(C - a - tt s)); +inf outside the open box */
+static double log_barrier(int n, const double* a, const double* s, double tt, double C) {
+    double acc = 0.0;
+    int bad = 0;
+    for (int i = 0; i < n; ++i) {
+        double v = a[i] + (s ? tt * s[i] : 0.0);
+        if (!(v > 0.0) || !(C - v > 0.0)) bad = 1;
+        else acc += log(v) + log(C - v);
+    }
+    return bad ? -INFINITY : acc;
+}
+
+int orc_svm_barrier(int n, int k, const double* Z, int ldz, const double* y, double C,
+                    double cg_tol, int max_outer, int max_newton, int max_cg, double* a,
+                    int* stats, double* fval) {
+    int plus = 0, minus = 0;
+    for (int i = 0; i < n; ++i) {
+        if (y[i] > 0) ++plus;
+        else ++minus;
+    }
+    if (plus == 0 || minus == 0) return -1;
+    double pval, mval; /* testsvm.cpp:204-211 */
+    if (plus > minus) {
+        mval = 0.9 * C;
+        pval = mval * minus / plus;
+    } else {
+        pval = 0.9 * C;
+        mval = pval * plus / minus;
+    }
+    for (int i = 0; i < n; ++i) a[i] = y[i] > 0 ? pval : mval;
+    double* w = (double*)malloc(sizeof(double) * (size_t)n * 9 + sizeof(double) * (size_t)k * 2);
+    double *M = w, *grad = w + n, *D = w + 2 * (size_t)n, *s = w + 3 * (size_t)n;
+    double *r = w + 4 * (size_t)n, *z = w + 5 * (size_t)n, *p = w + 6 * (size_t)n;
+    double *q = w + 7 * (size_t)n, *Pc = w + 8 * (size_t)n;
+    double *u = w + 9 * (size_t)n, *v = u + k;
+    for (int i = 0; i < n; ++i) { /* M = diag(Z Z^T), testsvm.cpp:236-243 */
+        double m2 = 0.0;
+        for (int c = 0; c < k; ++c) m2 += Z[i + (size_t)c * ldz] * Z[i + (size_t)c * ldz];
+        M[i] = m2;
+    }
+    double t = 1.0;
+    const double ls_alpha = 0.1, ls_beta = 0.5, mu = 20.0;
+    int outer = 0, newton = 0, cgit = 0;
+    double sum_a = 0.0, fcur = 0.0;
+    for (outer = 1; outer <= max_outer; ++outer) {
+        for (int ni = 1; ni <= max_newton; ++ni) {
+            /* gradient and barrier Hessian */
+            zt_x(n, k, Z, ldz, a, u);
+            z_u(n, k, Z, ldz, u, grad);
+            double dmax = 0.0, dmin = INFINITY;
+            for (int i = 0; i < n; ++i) {
+                double ai = a[i], ci = C - ai;
+                grad[i] += -1.0 + (-1.0 / ai + 1.0 / ci) / t;
+                D[i] = (1.0 / (ai * ai) + 1.0 / (ci * ci)) / t;
+                dmax = D[i] > dmax ? D[i] : dmax;
+                dmin = D[i] < dmin ? D[i] : dmin;
+            }
+            if (dmax / dmin > 1e18) break;
+            /* pre_conjgrad: (Z Z^T + diag(D)) s = -grad, Jacobi preconditioner */
+            for (int i = 0; i < n; ++i) {
+                s[i] = 0.0;
+                r[i] = -grad[i];
+                Pc[i] = M[i] + D[i];
+                z[i] = r[i] / Pc[i];
+                p[i] = z[i];
+            }
+            double rs = dot_n(n, r, z), normr = sqrt(dot_n(n, r, r));
+            for (int it = 0; normr > cg_tol && it < max_cg; ++it) {
+                zt_x(n, k, Z, ldz, p, v);
+                z_u(n, k, Z, ldz, v, q);
+                for (int i = 0; i < n; ++i) q[i] += D[i] * p[i];
+                double al = rs / dot_n(n, p, q);
+                for (int i = 0; i < n; ++i) {
+                    s[i] += al * p[i];
+                    r[i] -= al * q[i];
+                    z[i] = r[i] / Pc[i];
+                }
+                double rs_new = dot_n(n, r, z);
+                double be = rs_new / rs;
+                for (int i = 0; i < n; ++i) p[i] = z[i] + be * p[i];
+                rs = rs_new;
+                normr = sqrt(dot_n(n, r, r));
+                ++cgit;
+            }
+            ++newton;
+            /* line search on g(a + tt s) = f - log barrier / t; f through Z^T */
+            double gs = dot_n(n, grad, s);
+            double nd2 = -gs;
+            zt_x(n, k, Z, ldz, s, v);
+            sum_a = 0.0;
+            double sum_s = 0.0;
+            for (int i = 0; i < n; ++i) {
+                sum_a += a[i];
+                sum_s += s[i];
+            }
+            double uu = dot_n(k, u, u), uv = dot_n(k, u, v), vv = dot_n(k, v, v);
+            double g0 = 0.5 * uu - sum_a - log_barrier(n, a, NULL, 0.0, C) / t;
+            double tt = 1.0, gnew = g0;
+            for (int ls = 0; ls < 64; ++ls) {
+                double fn = 0.5 * (uu + 2.0 * tt * uv + tt * tt * vv) - (sum_a + tt * sum_s);
+                gnew = fn - log_barrier(n, a, s, tt, C) / t;
+                if (gnew <= g0 + ls_alpha * tt * gs) break;
+                tt *= ls_beta;
+            }
+            if (!(gnew <= g0 + ls_alpha * tt * gs)) tt = 0.0, gnew = g0;
+            for (int i = 0; i < n; ++i) a[i] += tt * s[i];
+            if (nd2 / 2 < 1e-6) break;
+            if (ni > 1 && (g0 - gnew) < 1e-9 * fabs(gnew)) break;
+        }
+        zt_x(n, k, Z, ldz, a, u);
+        sum_a = 0.0;
+        for (int i = 0; i < n; ++i) sum_a += a[i];
+        fcur = 0.5 * dot_n(k, u, u) - sum_a;
+        if (2.0 * n / t < fabs(fcur) * 1e-5) break;
+        t *= mu;
+    }
+    if (outer > max_outer) outer = max_outer;
+    if (stats) {
+        stats[0] = outer;
+        stats[1] = newton;
+        stats[2] = cgit;
+    }
+    if (fval) *fval = fcur;
+    free(w);
+    return 0;
+}

@@ -159,6 +159,33 @@ int orc_gesvj(int m, int n, double* A, int lda, double* sva, double* V, int ldv,
 int orc_nystrom(int n, int l, int k, const double* Q, int ldq, const double* W, int ldw,
                 double* Z, int ldz, double* lam);
 
+/* ---- the barrier SVM solver on the Nystrom factor (testsvm.cpp:191-321) ---- */
+
+/* Dual soft-margin SVM without the equality constraint, as the reference
+ * solves it: min f(a) = a^T Z Z^T a / 2 - sum(a) over 0 < a < C, Z (n x k)
+ * the Nystrom factor of diag(y) K diag(y), by the barrier method
+ * g(a) = f(a) - sum(log a_i + log(C - a_i)) / t (dbarrier.h:13-46):
+ *   a0: the smaller class at 0.9 C, the other scaled so that y^T a0 = 0
+ *       (testsvm.cpp:191-219);
+ *   Newton steps: gradient Z Z^T a - 1 + (-1/a + 1/(C-a))/t, barrier Hessian
+ *       D = (1/a^2 + 1/(C-a)^2)/t (dbarrier.h:48-80), step s from
+ *       (Z Z^T + diag(D)) s = -grad by Jacobi-preconditioned CG
+ *       (pre_conjgrad, dbarrier.h:83-139) from s = 0 until ||r|| <= cg_tol,
+ *       backtracking line search alpha = 0.1, beta = 0.5 on g, stop when
+ *       -grad.s / 2 < 1e-6, on slow progress (g decreasing by less than
+ *       1e-9 |g|), or when max D / min D > 1e18 (testsvm.cpp:240-303);
+ *   t *= 20 until 2n / t < 1e-5 |f(a)| (testsvm.cpp:307-314).
+ * Bug fixes against the reference: the CG right-hand side is -grad (the
+ * reference solves with b = e_1, testsvm.cpp:234-235), the operator has no
+ * added n I (dbarrier.h:99, 113), the preconditioner is diag(Z Z^T) + D, the
+ * CG starts from 0 (the reference starts from a, dbarrier.h:108), and the
+ * iteration counts are bounded.  Returns 0, or -1 if a class is empty.
+ * stats[0..2] = barrier iterations, Newton steps, CG iterations;
+ * stats (may be NULL) has 3 ints; *fval = f(a). */
+int orc_svm_barrier(int n, int k, const double* Z, int ldz, const double* y, double C,
+                    double cg_tol, int max_outer, int max_newton, int max_cg, double* a,
+                    int* stats, double* fval);
+
 /* Threads used by the OpenMP-parallel loops (1 = the sequential-BLAS link DArray uses). */
 void orc_set_threads(int nthreads);
 int orc_get_threads(void);

@@ -34,11 +34,14 @@ def test_driver_compiles_against_headers(name):
 def test_headers_declare_reference_entry_points():
     text = "".join((ROOT / "include" / "darray_hip" / f).read_text()
                    for f in ("darray.h", "dkernel.h", "blas_like.h", "lapack_like.h"))
+    text += "".join((ROOT / "include" / "darray_hip" / f).read_text()
+                    for f in ("dbarrier.h", "read_input.h"))
     for sym in ("class Grid", "class LMatrix", "class DMatrix", "struct DMatrixDescriptor",
                 "void LRA(", "double Knorm(", "void matrix_multiply(", "void triangular_solve(",
                 "void cholesky_factorize(", "void cholesky_factorize_recursive(",
                 "void multiply_matrix(", "void symmetric_update(", "void rbf_kernel(",
-                "void tc_gemm(", "void tc_sgemm("):
+                "void tc_gemm(", "void tc_sgemm(", "LMatrix<T> svd(", "void read_input_file(",
+                "void writemodel(", "int pre_conjgrad("):
         assert sym in text, sym
 
 
@@ -93,3 +96,11 @@ def test_svd_and_nystrom_factor_with_knorm():
     """DArray::svd known answers and the range finder -> Nystrom Z -> Knorm flow
     of testsvm.cpp:86-177 through the header mirror."""
     _run(["nystrom_knorm"], timeout=300)
+
+
+@pytest.mark.gpu
+def test_svm_pipeline_libsvm_to_model(tmp_path):
+    """testsvm.cpp's flow: LIBSVM file -> reader -> labelled range finder ->
+    Nystrom Z + Knorm -> barrier SVM -> LIBSVM model, with the training
+    accuracy of the written model's decision values."""
+    _run(["svm_pipeline", str(tmp_path)], timeout=300)

@@ -325,3 +325,45 @@ def test_nystrom_exact_for_low_rank_kernel():
     e2, k2 = orc.lra_error(X, 0.7, Z, Z)
     assert st == 0 and np.sqrt(e2 / k2) < 1e-12
     assert np.all(lam[r:] < 1e-10 * lam[0])  # eigenvalues past the rank vanish
+
+
+# ---------------------------------------------------------------- barrier SVM solver
+
+def _svm_problem(n, d, k, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(-1, 1, (n, d))
+    y = np.where(X[:, 0] + 0.3 * rng.standard_normal(n) > 0, 1.0, -1.0)
+    K = np.exp(-0.5 * ((X[:, None, :] - X[None, :, :]) ** 2).sum(-1))
+    w, U = np.linalg.eigh((y[:, None] * y[None, :]) * K)
+    idx = np.argsort(w)[::-1][:k]
+    return np.asfortranarray(U[:, idx] * np.sqrt(np.maximum(w[idx], 0))), y
+
+
+@pytest.mark.parametrize("n,d,k,Cb", [(300, 4, 40, 1.0), (800, 8, 60, 10.0)])
+def test_svm_barrier_reaches_the_box_qp_optimum(n, d, k, Cb):
+    """The barrier method stops when the gap bound 2n/t is below 1e-5 |f|, so
+    f(a) lies within 1e-5 |f*| of the box-constrained optimum, computed here
+    independently by L-BFGS-B; a stays strictly inside (0, C)."""
+    from scipy.optimize import minimize
+    Z, y = _svm_problem(n, d, k, n)
+    a, stats, fval, st = orc.svm_barrier(Z, y, Cb)
+    assert st == 0 and stats[0] >= 2 and stats[2] > 0
+    G = Z @ Z.T
+    assert abs(fval - (0.5 * a @ G @ a - a.sum())) <= 1e-10 * abs(fval)
+    res = minimize(lambda x: 0.5 * x @ G @ x - x.sum(), np.full(n, Cb / 2), jac=lambda x: G @ x - 1,
+                   method="L-BFGS-B", bounds=[(0, Cb)] * n,
+                   options=dict(ftol=1e-15, gtol=1e-12, maxiter=100000))
+    assert res.fun <= fval <= res.fun + 1e-5 * abs(res.fun)
+    assert np.all(a > 0) and np.all(a < Cb)
+
+
+def test_svm_barrier_start_point_and_empty_class():
+    # One Newton step with no CG iteration (s = 0): a stays at the start point,
+    # y^T a = 0 with the smaller class at 0.9 C (testsvm.cpp:204-219).
+    Z, y = _svm_problem(200, 3, 20, 3)
+    a, stats, _, st = orc.svm_barrier(Z, y, 2.0, max_outer=1, max_newton=1, max_cg=0)
+    assert st == 0 and stats == [1, 1, 0]
+    small = -1.0 if (y > 0).sum() > (y < 0).sum() else 1.0
+    assert np.all(a[y == small] == 0.9 * 2.0) and abs(y @ a) < 1e-9
+    _, _, _, st = orc.svm_barrier(Z, np.ones(200), 2.0)
+    assert st == -1
