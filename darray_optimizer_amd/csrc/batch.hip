@@ -2,8 +2,9 @@
 //
 // A batch of independent problems (n_i, d_i, l_i, shared q) runs stage by
 // stage, each stage one launch over a device table of all problems:
-//   prepare  : column means, BF16 centred points, squared norms
-//   per pass : pack the operand (Omega or Q) -> persistent BF16 projection
+//   prepare  : column means, 16-bit centred points (FP16: scaled by a power of
+//              two found on the device), squared norms
+//   per pass : pack the operand (Omega or Q) -> persistent 16-bit projection
 //              kernels, one per tile class (dpad, column blocks), pulling
 //              (problem, row block) items from an atomic queue, longest
 //              problem first -> CholeskyQR2 (batched split-K Gram, one
@@ -229,7 +230,8 @@ int tune_classes(dh_handle_t h, std::vector<ClassRange>& classes, const DhProjJo
                                                 counters);
             continue;
         }
-        uint64_t key = dh_tune_key_bf16(cr.cls.dk, cr.cls.ncb, 1, cr.count * 256) ^ ((uint64_t)1 << 60);
+        uint64_t key = dh_tune_key_bf16(cr.cls.dk, cr.cls.ncb, 1, cr.count * 256) ^ ((uint64_t)1 << 60) ^
+                       ((uint64_t)cr.cls.f16 << 61);
         int known = dh_tune_lookup(h, key);
         if (known < 0 && !dh_tune_enabled()) known = DH_BF16_DEFAULT_WIDE;
         // Candidates run once eagerly (their launch set-up then happens outside
@@ -288,7 +290,7 @@ int build_plan(dh_handle_t h, Plan& P, int prec, int count, const dh_lra_problem
     const size_t o_gram = cv.take(sizeof(GramJob) * count * 2);
     const size_t o_spd = cv.take(sizeof(DhSpdJob) * count * 2);
     const size_t o_copy = cv.take(sizeof(CopyJob) * count);
-    const size_t o_info = cv.take(sizeof(int) * count * 3);
+    const size_t o_info = cv.take(sizeof(int) * count * 4);  // statuses, sticky, FP16 maxabs
     int max_n = 0, max_d = 0, max_npad = 0, max_l = 0, max_tri = 0, max_split = 0;
     long max_pack_threads = 0, max_nn = 0, max_nl = 0;
     size_t nitems = 0;
@@ -335,7 +337,8 @@ int build_plan(dh_handle_t h, Plan& P, int prec, int count, const dh_lra_problem
             std::stable_sort(ids.begin(), ids.end(),
                              [&](int a, int b) { return probs[a].n > probs[b].n; });
             ClassRange cr;
-            cr.cls = {kv.first.first, kv.first.second, dh_bf16_rows(kv.first.first * 32), 0};
+            cr.cls = {kv.first.first, kv.first.second, dh_bf16_rows(kv.first.first * 32), 0,
+                      prec == DH_PREC_F16 ? 1 : 0};
             cr.first = (int)items.size();
             for (int i : ids)
                 for (int r = 0; r < in[i].rows; ++r)
@@ -365,7 +368,8 @@ int build_plan(dh_handle_t h, Plan& P, int prec, int count, const dh_lra_problem
         const Inst& s = in[i];
         const double neg = -p.gamma * (double)kLog2e;
         sets[i] = {p.n, p.d, s.dpad, s.npad, p.X, p.ldx, neg, (double*)at(s.o_part),
-                   (double*)at(s.o_mean), at(s.o_xb), (float*)at(s.o_a)};
+                   (double*)at(s.o_mean), at(s.o_xb), (float*)at(s.o_a), p.gamma, 1.0,
+                   (unsigned*)(P.infos + 3 * count + i)};
         for (int t = 0; t < 2; ++t) {
             pack[t * count + i] = {p.n, p.l, s.nchunks, s.ncb_pack, t == 0 ? p.Omega : p.Q,
                                    t == 0 ? p.ldo : p.ldq, p.y, at(s.o_op)};
@@ -396,7 +400,8 @@ int build_plan(dh_handle_t h, Plan& P, int prec, int count, const dh_lra_problem
 
     const DhBf16Set* d_sets = (const DhBf16Set*)at(o_sets);
     const DhPackJob* d_pack = (const DhPackJob*)at(o_pack);
-    const DhProjJob* d_proj = (const DhProjJob*)at(o_proj);
+    DhProjJob* d_proj = (DhProjJob*)at(o_proj);
+    const bool f16 = prec == DH_PREC_F16;
     const GramJob* d_gram = (const GramJob*)at(o_gram);
     const DhSpdJob* d_spd = (const DhSpdJob*)at(o_spd);
     const CopyJob* d_copy = (const CopyJob*)at(o_copy);
@@ -405,8 +410,8 @@ int build_plan(dh_handle_t h, Plan& P, int prec, int count, const dh_lra_problem
     const int ncounters = passes * (int)classes.size();
 
     // ---- eager preparation + autotuning of the projection schedules
-    DH_CHECK_STATUS(dh_bf16_prepare_batched(h, d_sets, count, max_n, max_d, max_npad));
-    DH_CHECK_STATUS(dh_bf16_pack_batched(h, d_pack, count, max_pack_threads));
+    DH_CHECK_STATUS(dh_bf16_prepare_batched(h, d_sets, count, max_n, max_d, max_npad, f16, d_proj));
+    DH_CHECK_STATUS(dh_bf16_pack_batched(h, d_pack, count, max_pack_threads, f16));
     DH_CHECK_STATUS(tune_classes(h, classes, d_proj, d_items, d_counters));
     // Warm the batched CholeskyQR launcher (it sets a kernel attribute once).
     DH_CHECK_STATUS(dh_cholqr_step_batched(h, d_spd, 0, max_l, max_n, false));
@@ -430,10 +435,12 @@ int build_plan(dh_handle_t h, Plan& P, int prec, int count, const dh_lra_problem
     do {
         fill_int_kernel<<<dh_cdiv(ncounters, 256), 256, 0, h->stream>>>(d_counters, ncounters, 0);
         fill_int_kernel<<<dh_cdiv(3 * count, 256), 256, 0, h->stream>>>(P.infos, 3 * count, 0);
-        if ((st = dh_bf16_prepare_batched(h, d_sets, count, max_n, max_d, max_npad))) break;
+        if ((st = dh_bf16_prepare_batched(h, d_sets, count, max_n, max_d, max_npad, f16, d_proj)))
+            break;
         for (int pass = 0; pass < passes && !st; ++pass) {
             const int t = pass == 0 ? 0 : 1;  // operand Omega / Q, output Q / W
-            if ((st = dh_bf16_pack_batched(h, d_pack + t * count, count, max_pack_threads))) break;
+            if ((st = dh_bf16_pack_batched(h, d_pack + t * count, count, max_pack_threads, f16)))
+                break;
             for (size_t c = 0; c < classes.size() && !st; ++c)
                 st = dh_bf16_project_persistent(h, classes[c].cls, d_proj + t * count,
                                                 d_items + classes[c].first, classes[c].count,
@@ -479,12 +486,13 @@ extern "C" int dh_lra_batched(dh_handle_t h, int prec, int count, const dh_lra_p
             p.ldq < p.n || p.ldw < p.n || !p.X || !p.Omega || !p.Q || !p.W)
             return dh_fail(-4, "%s: problem %d: bad sizes or pointers (n=%d d=%d l=%d)", name, i,
                            p.n, p.d, p.l);
-        if (prec == DH_PREC_BF16 && (p.d > 256 || p.l > 272))
-            return dh_fail(-4, "%s: problem %d: BF16 batches need d <= 256 and l <= 272", name, i);
+        if ((prec == DH_PREC_BF16 || prec == DH_PREC_F16) && (p.d > 256 || p.l > 272))
+            return dh_fail(-4, "%s: problem %d: 16-bit batches need d <= 256 and l <= 272", name,
+                           i);
     }
     std::vector<int> st(count, 0);
     int first = 0;
-    if (prec != DH_PREC_BF16) {
+    if (prec != DH_PREC_BF16 && prec != DH_PREC_F16) {
         for (int i = 0; i < count; ++i) {
             const dh_lra_problem& p = problems[i];
             st[i] = dh_lra(h, prec, p.n, p.d, p.l, p.q, p.X, p.ldx, p.y, p.gamma, p.Omega,

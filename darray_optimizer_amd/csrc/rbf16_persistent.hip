@@ -1,4 +1,5 @@
-// Batched persistent projection kernels (dh_lra_batched): one per tile class.
+// Batched persistent projection kernels (dh_lra_batched): one per tile class
+// and operand type (BF16 or FP16).
 #include "batch.h"
 #include "rbf16_kernels.h"
 
@@ -7,7 +8,7 @@ namespace {
 // Persistent workgroups: each takes the next item from *counter until the
 // queue is empty; every wave of the workgroup sees the same item and leaves
 // the loop together.  An item is 256 rows (128 for d > 128) of one job.
-template <int DK, int NCB, int WAVES, int RB>
+template <int DK, int NCB, int WAVES, int RB, bool F16>
 __global__ __launch_bounds__(WAVES * 64, 1) void rbf_project_bf16_persistent(
     const DhProjJob* __restrict__ jobs, const DhProjItem* __restrict__ items, int nitems,
     int* __restrict__ counter) {
@@ -20,7 +21,7 @@ __global__ __launch_bounds__(WAVES * 64, 1) void rbf_project_bf16_persistent(
         if (it >= nitems) break;
         const DhProjItem w = items[it];
         const DhProjJob& j = jobs[w.job];
-        project_rows<DK, NCB, WAVES, false, RB>(w.rblk, w.cg, j.m, j.nchunks, j.l, j.ncb_pack,
+        project_rows<DK, NCB, WAVES, F16, RB>(w.rblk, w.cg, j.m, j.nchunks, j.l, j.ncb_pack,
                                                 (const bf16x8*)j.xb, (const bf16x8*)j.xb, j.a, j.a,
                                                 nullptr, nullptr, j.c2, (const bf16x8*)j.Op, j.y, j.Y,
                                                 j.ldy);
@@ -28,9 +29,10 @@ __global__ __launch_bounds__(WAVES * 64, 1) void rbf_project_bf16_persistent(
 }
 
 template <int DK, int NCB, int W, int RB>
-int launch_persistent(dh_handle_t h, const DhProjJob* jobs, const DhProjItem* items, int nitems,
-                      int* counter) {
-    auto kern = rbf_project_bf16_persistent<DK, NCB, W, RB>;
+int launch_persistent(dh_handle_t h, bool f16, const DhProjJob* jobs, const DhProjItem* items,
+                      int nitems, int* counter) {
+    auto kern = f16 ? rbf_project_bf16_persistent<DK, NCB, W, RB, true>
+                    : rbf_project_bf16_persistent<DK, NCB, W, RB, false>;
     int per_cu = 1;
     DH_CHECK_STATUS(dh_occupancy(h, (const void*)kern, W * 64, &per_cu));
     int grid = max(1, min(nitems, per_cu * h->num_cu));
@@ -40,15 +42,15 @@ int launch_persistent(dh_handle_t h, const DhProjJob* jobs, const DhProjItem* it
 }
 
 template <int DK>
-int persistent_ncb(dh_handle_t h, int ncb, bool wide, const DhProjJob* jobs,
+int persistent_ncb(dh_handle_t h, int ncb, bool wide, bool f16, const DhProjJob* jobs,
                    const DhProjItem* items, int nitems, int* counter) {
-#define DH_PERS(N)                                                                            \
-    case N:                                                                                   \
-        if constexpr (DK >= 8)                                                                \
-            return launch_persistent<DK, N, 4, 2>(h, jobs, items, nitems, counter);           \
-        else                                                                                  \
-            return wide ? launch_persistent<DK, N, 4, 4>(h, jobs, items, nitems, counter)     \
-                        : launch_persistent<DK, N, 8, 2>(h, jobs, items, nitems, counter);
+#define DH_PERS(N)                                                                             \
+    case N:                                                                                    \
+        if constexpr (DK >= 8)                                                                 \
+            return launch_persistent<DK, N, 4, 2>(h, f16, jobs, items, nitems, counter);       \
+        else                                                                                   \
+            return wide ? launch_persistent<DK, N, 4, 4>(h, f16, jobs, items, nitems, counter) \
+                        : launch_persistent<DK, N, 8, 2>(h, f16, jobs, items, nitems, counter);
     switch (ncb) {
         DH_PERS(1)
         DH_PERS(2)
@@ -68,12 +70,12 @@ int persistent_ncb(dh_handle_t h, int ncb, bool wide, const DhProjJob* jobs,
 int dh_bf16_project_persistent(dh_handle_t h, DhBf16Class c, const DhProjJob* jobs,
                                const DhProjItem* items, int nitems, int* counter) {
     if (nitems == 0) return 0;
-    const bool wide = c.wide != 0;
+    const bool wide = c.wide != 0, f16 = c.f16 != 0;
     switch (c.dk) {
-        case 1: return persistent_ncb<1>(h, c.ncb, wide, jobs, items, nitems, counter);
-        case 2: return persistent_ncb<2>(h, c.ncb, wide, jobs, items, nitems, counter);
-        case 4: return persistent_ncb<4>(h, c.ncb, wide, jobs, items, nitems, counter);
-        case 8: return persistent_ncb<8>(h, c.ncb, false, jobs, items, nitems, counter);
+        case 1: return persistent_ncb<1>(h, c.ncb, wide, f16, jobs, items, nitems, counter);
+        case 2: return persistent_ncb<2>(h, c.ncb, wide, f16, jobs, items, nitems, counter);
+        case 4: return persistent_ncb<4>(h, c.ncb, wide, f16, jobs, items, nitems, counter);
+        case 8: return persistent_ncb<8>(h, c.ncb, false, f16, jobs, items, nitems, counter);
         default: return dh_fail(DH_ERR_UNSUPPORTED, "rbf_project_bf16: feature blocks %d", c.dk);
     }
 }

@@ -705,18 +705,68 @@ __global__ __launch_bounds__(256) void colmean_final_batched_kernel(const DhBf16
     }
 }
 
+template <bool F16>
 __global__ __launch_bounds__(256) void convert_rows_batched_kernel(const DhBf16Set* __restrict__ sets) {
     const DhBf16Set& s = sets[blockIdx.y];
     int row = (blockIdx.x * 256 + threadIdx.x) >> 6, lane = threadIdx.x & 63;
     if (row >= s.npad) return;
-    convert_row<false>(row, lane, s.n, s.d, s.dpad, s.X, s.ldx, s.mean, 1.0, (float)s.neg_gl2e,
-                       (__bf16*)s.xb, s.a);
+    convert_row<F16>(row, lane, s.n, s.d, s.dpad, s.X, s.ldx, s.mean, F16 ? s.scale : 1.0,
+                     (float)s.neg_gl2e, (__bf16*)s.xb, s.a);
 }
 
+template <bool F16>
 __global__ __launch_bounds__(256) void pack_batched_kernel(const DhPackJob* __restrict__ jobs) {
     const DhPackJob& j = jobs[blockIdx.y];
-    pack_one<false>((long)blockIdx.x * 256 + threadIdx.x, j.n, j.src ? j.l : 0, j.nchunks,
-                    j.ncb_pack, j.src, j.lds, j.y, (bf16x8*)j.Op);
+    pack_one<F16>((long)blockIdx.x * 256 + threadIdx.x, j.n, j.src ? j.l : 0, j.nchunks,
+                  j.ncb_pack, j.src, j.lds, j.y, (bf16x8*)j.Op);
+}
+
+// Per set: *maxabs = 0 (before maxabs_batched_kernel).
+__global__ void clear_maxabs_batched_kernel(DhBf16Set* __restrict__ sets, int count) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i < count) *sets[i].maxabs = 0u;
+}
+
+// max |X - mean| of set blockIdx.y (grid-strided over its elements; one
+// atomic per workgroup).
+__global__ __launch_bounds__(256) void maxabs_batched_kernel(const DhBf16Set* __restrict__ sets) {
+    const DhBf16Set& s = sets[blockIdx.y];
+    float mx = 0.f;
+    const long total = (long)s.n * s.d;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+        const int i = (int)(e / s.d), f = (int)(e % s.d);
+        mx = fmaxf(mx, (float)fabs(s.X[f + (size_t)i * s.ldx] - s.mean[f]));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    __shared__ float wmax[4];
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicMax(s.maxabs, __float_as_uint(fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]))));
+}
+
+// Per set: scale = 2^-e with max|x - mean| = f 2^e, f in [1/2, 1); the gamma of
+// the scaled points gamma / scale^2 goes into the set (-gamma log2 e) and into
+// the c2 = 2 gamma log2 e of both projection jobs of the problem.
+__global__ void f16_scale_batched_kernel(DhBf16Set* __restrict__ sets, DhProjJob* __restrict__ proj,
+                                         int count) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= count) return;
+    DhBf16Set& s = sets[i];
+    const float mx = __uint_as_float(*s.maxabs);
+    double scale = 1.0;
+    if (mx > 0.f && isfinite(mx)) {
+        int e = 0;
+        (void)frexp((double)mx, &e);
+        scale = ldexp(1.0, -e);
+    }
+    const double g = s.gamma / (scale * scale);
+    s.scale = scale;
+    s.neg_gl2e = -g * (double)kLog2e;
+    const float c2 = (float)(2.0 * g) * kLog2e;
+    proj[i].c2 = c2;
+    proj[count + i].c2 = c2;
 }
 
 }  // namespace
@@ -736,20 +786,38 @@ size_t dh_bf16_packed_bytes(int n, int ncb_pack) {
 }
 
 int dh_bf16_prepare_batched(dh_handle_t h, const DhBf16Set* sets, int count, int max_n, int max_d,
-                            int max_npad) {
+                            int max_npad, bool f16, DhProjJob* proj) {
     if (count == 0) return 0;
     colsum_partial_batched_kernel<<<dim3(dh_cdiv(max(max_n, 1), 256), count), 256, 0, h->stream>>>(sets);
     DH_CHECK_LAUNCH();
     colmean_final_batched_kernel<<<dim3(dh_cdiv(max_d, 256), count), 256, 0, h->stream>>>(sets);
     DH_CHECK_LAUNCH();
-    convert_rows_batched_kernel<<<dim3(dh_cdiv(max(max_npad, 1), 4), count), 256, 0, h->stream>>>(sets);
+    if (f16) {
+        DhBf16Set* msets = const_cast<DhBf16Set*>(sets);
+        clear_maxabs_batched_kernel<<<dh_cdiv(count, 64), 64, 0, h->stream>>>(msets, count);
+        DH_CHECK_LAUNCH();
+        const int blocks = max(1, min(dh_cdiv((long)max_n * max_d, 256), 64));
+        maxabs_batched_kernel<<<dim3(blocks, count), 256, 0, h->stream>>>(sets);
+        DH_CHECK_LAUNCH();
+        f16_scale_batched_kernel<<<dh_cdiv(count, 64), 64, 0, h->stream>>>(msets, proj, count);
+        DH_CHECK_LAUNCH();
+        convert_rows_batched_kernel<true>
+            <<<dim3(dh_cdiv(max(max_npad, 1), 4), count), 256, 0, h->stream>>>(sets);
+    } else {
+        convert_rows_batched_kernel<false>
+            <<<dim3(dh_cdiv(max(max_npad, 1), 4), count), 256, 0, h->stream>>>(sets);
+    }
     DH_CHECK_LAUNCH();
     return 0;
 }
 
-int dh_bf16_pack_batched(dh_handle_t h, const DhPackJob* jobs, int count, long max_threads) {
+int dh_bf16_pack_batched(dh_handle_t h, const DhPackJob* jobs, int count, long max_threads,
+                         bool f16) {
     if (count == 0 || max_threads == 0) return 0;
-    pack_batched_kernel<<<dim3(dh_cdiv(max_threads, 256), count), 256, 0, h->stream>>>(jobs);
+    if (f16)
+        pack_batched_kernel<true><<<dim3(dh_cdiv(max_threads, 256), count), 256, 0, h->stream>>>(jobs);
+    else
+        pack_batched_kernel<false><<<dim3(dh_cdiv(max_threads, 256), count), 256, 0, h->stream>>>(jobs);
     DH_CHECK_LAUNCH();
     return 0;
 }

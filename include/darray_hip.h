@@ -247,7 +247,8 @@ typedef struct dh_lra_problem {
 
 /* dh_lra for `count` independent problems of any n, d <= 256 and l <= 272
  * (one-vs-rest or per-gamma SVM problems; no reference counterpart: DArray
- * runs one LRA per process, testsvm.cpp:86-120).  With DH_PREC_BF16 every
+ * runs one LRA per process, testsvm.cpp:86-120).  With DH_PREC_BF16 or
+ * DH_PREC_F16 (each problem's power-of-two scale found on the device) every
  * stage runs as one launch over all problems -- point preparation, operand
  * packing, CholeskyQR2 (batched Gram, one-workgroup Cholesky per problem,
  * row-parallel solve) -- and the K-passes as persistent kernels, one per
@@ -257,7 +258,7 @@ typedef struct dh_lra_problem {
  * same.  Tile schedules come from the autotuner.  A problem whose
  * CholeskyQR breaks down is redone alone by dh_lra (shifted CholeskyQR3,
  * completion of rank-deficient bases).
- * DH_PREC_F64, DH_PREC_F16 and DH_PREC_FP8 run dh_lra per problem.  status (host, may be NULL) receives
+ * DH_PREC_F64 and DH_PREC_FP8 run dh_lra per problem.  status (host, may be NULL) receives
  * each problem's dh_lra status; returns 0 or the first non-zero status.
  * All problems must share q.  Synchronises. */
 int dh_lra_batched(dh_handle_t h, int prec, int count, const dh_lra_problem* problems,

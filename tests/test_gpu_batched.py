@@ -1,6 +1,6 @@
 """Batched range finder (dh_lra_batched) against dh_lra run problem by problem.
 
-The BF16 batch shares every kernel body with the single-problem path; only
+The BF16 and FP16 batches share every kernel body with the single-problem path; only
 the CholeskyQR Gram matrices are summed in another order, so the batched
 Q W^T agrees with the single-problem one to BF16 re-rounding level and the
 approximation errors agree closely.  The FP64 batch calls dh_lra per problem
@@ -49,15 +49,19 @@ def rel_err(dev, p, Q, W):
     return float(np.sqrt(max(e2, 0.0) / k2))
 
 
-def test_bf16_batch_matches_single_problems(dev):
+@pytest.mark.parametrize("prec", [dh.DH_PREC_BF16, dh.DH_PREC_F16], ids=["bf16", "f16"])
+def test_16bit_batch_matches_single_problems(dev, prec):
+    """The FP16 batch finds each problem's power-of-two scale on the device
+    (the single-problem path reads it back to the host): same scale, same
+    rounding, so the same bar as BF16 holds."""
     import torch
     probs = problems()
     print("batch:", [(p["X"].shape[0], p["X"].shape[1], p["Omega"].shape[0]) for p in probs],
           flush=True)
-    r, st = dev.lra_batched(probs, prec=dh.DH_PREC_BF16)
+    r, st = dev.lra_batched(probs, prec=prec)
     assert r == 0 and st == [0] * len(probs), st
     for i, p in enumerate(probs):
-        s1, Q1, W1 = single(dev, p, dh.DH_PREC_BF16)
+        s1, Q1, W1 = single(dev, p, prec)
         assert s1 == 0
         Q2, W2 = p["Q"], p["W"]
         l = Q2.shape[0]
@@ -72,26 +76,30 @@ def test_bf16_batch_matches_single_problems(dev):
         assert abs(e2 - e1) <= 0.02 * e1 + 1e-6
 
 
-def test_bf16_graph_replay_and_inputs_changed_in_place(dev):
+@pytest.mark.parametrize("prec", [dh.DH_PREC_BF16, dh.DH_PREC_F16], ids=["bf16", "f16"])
+def test_16bit_graph_replay_and_inputs_changed_in_place(dev, prec):
+    """Replays read the current inputs: for FP16 the scale is recomputed inside
+    the graph, so rescaled points (X * 3) give the same result as a fresh plan."""
     import torch
     probs = problems(shapes=SHAPES[:5])
-    dev.lra_batched(probs)
+    dev.lra_batched(probs, prec=prec)
     first = [(p["Q"].clone(), p["W"].clone()) for p in probs]
     for rep in range(3):  # replays of the captured graph, with eager work in between
         torch.cuda.synchronize()
         junk = torch.randn(1000, device="cuda").sum().item()
-        r, st = dev.lra_batched(probs)
+        r, st = dev.lra_batched(probs, prec=prec)
         assert r == 0
         for (Q0, W0), p in zip(first, probs):
             assert torch.equal(Q0, p["Q"]) and torch.equal(W0, p["W"]), f"replay {rep} differs"
     # New data in the same buffers: the replay must read it.
     for p in probs:
         p["Omega"].mul_(-0.5).add_(0.25)
-    dev.lra_batched(probs)
+        p["X"].mul_(3.0)
+    dev.lra_batched(probs, prec=prec)
     replayed = [(p["Q"].clone(), p["W"].clone()) for p in probs]
     fresh = dh.Handle(0)
     try:
-        fresh.lra_batched(probs)
+        fresh.lra_batched(probs, prec=prec)
         for (Qr, Wr), p in zip(replayed, probs):
             assert torch.equal(Qr, p["Q"]) and torch.equal(Wr, p["W"])
     finally:
@@ -129,3 +137,5 @@ def test_batched_argument_errors(dev):
     probs[1]["q"] = 2
     with pytest.raises(dh.DHError):
         dev.lra_batched(probs)
+    with pytest.raises(dh.DHError):
+        dev.lra_batched(probs, prec=dh.DH_PREC_F16)
