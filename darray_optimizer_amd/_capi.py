@@ -40,6 +40,8 @@ SIGNATURES: dict[str, tuple] = {
     "dh_rbf_tile_f32": (_i, [_p, _i, _i, _i, _p, _i, _p, _i, _p, _p, _f, _p, _i]),
     "dh_rbf_project_f64": (_i, [_p, _i, _i, _i, _i, _p, _i, _p, _i, _p, _p, _d, _p, _i, _d, _p,
                                 _i]),
+    "dh_rbf_project_f64_noalloc": (_i, [_p, _i, _i, _i, _i, _p, _i, _p, _i, _p, _p, _d, _p, _i,
+                                        _d, _p, _i]),
     "dh_rbf_project_bf16": (_i, [_p, _i, _i, _i, _i, _p, _i, _p, _i, _p, _p, _d, _p, _i, _p,
                                  _i]),
     "dh_rbf_project_f16": (_i, [_p, _i, _i, _i, _i, _p, _i, _p, _i, _p, _p, _d, _p, _i, _p,

@@ -34,7 +34,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall",
 
 SOURCES = ["handle.hip", "dense.hip", "small_spd.hip", "solve.hip", "rbf_f64.hip", "rbf_bf16.hip",
            "rbf16_dk1.hip", "rbf16_dk2.hip", "rbf16_dk4.hip", "rbf16_dk8.hip",
-           "rbf16_fp8.hip", "rbf16_persistent.hip", "lra.hip", "batch.hip", "nystrom.hip", "svm.hip", "gen.cpp", "libsvm_io.cpp", "shim.cpp"]
+           "rbf16_fp8.hip", "rbf16_persistent.hip", "lra.hip", "batch.hip", "noalloc.hip", "nystrom.hip", "svm.hip", "gen.cpp", "libsvm_io.cpp", "shim.cpp"]
 
 
 def _compile(src: str) -> Path:

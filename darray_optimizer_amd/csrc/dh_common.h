@@ -56,6 +56,10 @@ struct dh_handle_s {
 // Creates h->aux and its events if they do not exist yet.
 int dh_aux_stream(dh_handle_t h);
 
+// Looks up, once per device, the device address of the static scratch array
+// of dh_rbf_project_f64_noalloc (noalloc.hip); `device` must be current.
+int dh_noalloc_setup(int device);
+
 // *per_cu = resident workgroups per CU of `kernel` at `threads` threads per
 // workgroup on the handle's device (at least 1), cached in the handle.
 int dh_occupancy(dh_handle_t h, const void* kernel, int threads, int* per_cu);

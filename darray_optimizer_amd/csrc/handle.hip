@@ -62,6 +62,7 @@ extern "C" int dh_create(int device, void* stream, dh_handle_t* out) {
     if (device < 0 || device >= ndev)
         return dh_fail(-1, "dh_create: device %d out of range (%d devices)", device, ndev);
     DH_CHECK_HIP(hipSetDevice(device));
+    DH_CHECK_STATUS(dh_noalloc_setup(device));
     dh_handle_s* h = new (std::nothrow) dh_handle_s();
     if (!h) return dh_fail(DH_ERR_ALLOC, "dh_create: out of host memory");
     h->device = device;

@@ -106,6 +106,22 @@ int dh_rbf_project_f64(dh_handle_t h, int m, int n, int d, int l, const double* 
                        const double* Xj, int ldxj, const double* yi, const double* yj,
                        double gamma, const double* O, int ldo, double beta, double* Y, int ldy);
 
+/* Allocation-free variant of dh_rbf_project_f64, same arguments and results.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  Its only scratch, the squared norms of Xi and Xj, lives
+ * in a static device array of DH_NOALLOC_MAX_POINTS doubles (8 MiB, held by
+ * every GPU the library runs on until the process ends; dh_create looks up
+ * its address once per device), so m + n <= DH_NOALLOC_MAX_POINTS, else
+ * DH_ERR_UNSUPPORTED before any launch.  The call makes no hipMalloc,
+ * hipFree or other allocation.  Calls that overlap on one GPU (any handle,
+ * any stream) race on the array. */
+#define DH_NOALLOC_MAX_POINTS (1 << 20)
+int dh_rbf_project_f64_noalloc(dh_handle_t h, int m, int n, int d, int l, const double* Xi,
+                               int ldxi, const double* Xj, int ldxj, const double* yi,
+                               const double* yj, double gamma, const double* O, int ldo,
+                               double beta, double* Y, int ldy);
+
 /* As dh_rbf_project_bf16 with FP16 in place of BF16 (points also scaled by a
  * power of two so that max |x - mean| lies in (1/2, 1]; one synchronisation):
  * the same MFMA rate with 3 more mantissa bits for the points, K and O. */
