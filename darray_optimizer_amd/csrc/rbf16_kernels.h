@@ -287,13 +287,9 @@ __device__ __forceinline__ void project_rows(
         }
     };
     // Projection B operands go through a ring of PF registers: the read of
-    // column block cb + PF is issued before the MFMAs of block cb.  Eight
-    // reads in flight measured 5% faster than four at d = 128, l = 272 (the
-    // 16x16x32 MFMA drains a four-deep ring before the LDS latency is over);
-    // the wider point sets and the 64-row waves keep four (their Xi fragments
-    // and accumulators take the registers).
-    constexpr int PFW = DK <= 4 && RB <= 2 && !D8 ? 8 : 4;
-    constexpr int PF = NCB < PFW ? NCB : PFW;
+    // column block cb + PF is issued before the MFMAs of block cb (an eight-deep
+    // ring measured the same at d = 128, l = 272).
+    constexpr int PF = NCB < 4 ? NCB : 4;
     auto project_start = [&](int buf, bf16x8 (&ring)[PF]) {
 #pragma unroll
         for (int p = 0; p < PF; ++p) ring[p] = sO[buf][p * 64 + lane];

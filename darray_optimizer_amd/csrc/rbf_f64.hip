@@ -133,13 +133,24 @@ extern "C" int dh_rbf_tile_f32(dh_handle_t h, int m, int n, int d, const float* 
 //
 // Workgroup: 4 waves, 64 rows i (16 per wave).  The j loop walks Xj and
 // Omega in chunks of 16 points staged in LDS.  Per chunk each wave
-//   1. forms S^T = Xj_chunk * Xi_wave^T (16 j x 16 i) with DK f64 MFMAs,
-//      A operand from LDS (Xj^T, f-major), B operand from registers (Xi);
+//   1. forms S^T = Xj_chunk * Xi_wave^T (16 j x 16 i) with DK f64 MFMAs in two
+//      independent accumulation chains.  Feature f = g*DK + kk is k-slot g of
+//      MFMA kk, so lane (g, li) reads its A operands (point li of the chunk)
+//      as consecutive doubles of a point-major LDS row, two per 16-byte read,
+//      and keeps its B operands (row li of Xi, features g*DK ..) in registers;
 //   2. turns its 4 accumulators into K values in place (exp epilogue);
 //   3. uses them directly as the A operand of the projection MFMAs: lane L
 //      holds S^T[L/16+4r][L%16] = K[i=L%16][j=4r+L/16] (f64 accumulator
 //      layout), i.e. MFMA r contracts over j = 4r..4r+3; Omega is staged in
-//      LDS in the matching order sO[r][cb][L] = Omega[j0+4r+L/16][c0+16cb+L%16].
+//      LDS so that lane L reads the B operands of MFMAs 2h and 2h+1 of block
+//      cb as one 16-byte pair sO[cb][h][L] = (Omega[j0+4(2h)+L/16][c],
+//      Omega[j0+4(2h+1)+L/16][c]), c = c0 + 16cb + L%16.
+// The chunks go through two LDS buffers: the global loads of chunk t+1 are
+// issued into registers before chunk t is computed and stored to the other
+// buffer after it, so one barrier per chunk orders both.  (With one buffer,
+// single 8-byte operand reads consumed at once and the accumulators moved
+// between register files every chunk, the pass ran at 38% of the FP64 MFMA
+// peak; this layout runs at 51%.)
 // Labels are folded: yj scales Omega as it is staged, yi scales the output.
 // The projection accumulators (NCB blocks of 16 columns) stay in registers
 // for the whole j loop; Y is written once.
@@ -150,11 +161,13 @@ __global__ __launch_bounds__(256) void rbf_project_f64_kernel(
     const double* __restrict__ nj, const double* __restrict__ yi, const double* __restrict__ yj,
     double gamma, const double* __restrict__ O, int ldo, double beta, double* __restrict__ Y,
     int ldy) {
+    typedef double d2v __attribute__((ext_vector_type(2)));
     constexpr int JB = 16;
     constexpr int FD = DK * 4;  // padded feature count
-    __shared__ double sX[FD][JB];
-    __shared__ double sO[4][NCB][64];
-    __shared__ double sN[JB];
+    constexpr int XP = FD + 2;  // LDS pitch of a point (odd multiple of 16 bytes: no conflicts)
+    __shared__ double sX[2][JB * XP];
+    __shared__ d2v sO[2][NCB][2][64];
+    __shared__ double sN[2][JB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, li = lane & 15;
     const int c0 = blockIdx.y * NCB * 16;
@@ -164,7 +177,7 @@ __global__ __launch_bounds__(256) void rbf_project_f64_kernel(
     double xr[DK];
 #pragma unroll
     for (int kk = 0; kk < DK; ++kk) {
-        int f = 4 * kk + g;
+        const int f = g * DK + kk;
         xr[kk] = (row < m && f < d) ? Xi[f + (size_t)row * ldxi] : 0.0;
     }
     const double nrow = (row < m) ? ni[row] : 0.0;
@@ -174,58 +187,77 @@ __global__ __launch_bounds__(256) void rbf_project_f64_kernel(
 
     // Staging of a chunk: every load is issued before any LDS store, from
     // addresses clamped into the operands (a load under a branch gets a wait
-    // of its own and the chunk's ~25 loads per thread would run one after
-    // another).  Thread tid always stages point jj = tid % JB of the Omega
+    // of its own).  Thread tid always stages point jj = tid % JB of the Omega
     // block, so its label is one load per chunk.
     constexpr int NXL = JB * FD / 256, NOL = JB * NCB * 16 / 256;
     static_assert(JB * FD % 256 == 0 && JB * NCB * 16 % 256 == 0, "staging split");
-    for (int j0 = 0; j0 < n; j0 += JB) {
-        double px[NXL], po[NOL];
+    double px[NXL], po[NOL], pn, yv;
+    auto load = [&](int j0) {
 #pragma unroll
         for (int u = 0; u < NXL; ++u) {
             const int e = tid + 256 * u, jj = e / FD, f = e % FD;
             px[u] = Xj[min(f, d - 1) + (size_t)min(j0 + jj, n - 1) * ldxj];
         }
         const int jo = j0 + (tid % JB);
-        const double yv = yj ? yj[min(jo, n - 1)] : 1.0;
+        yv = yj ? yj[min(jo, n - 1)] : 1.0;
 #pragma unroll
         for (int u = 0; u < NOL; ++u) {
             const int e = tid + 256 * u, cl = e / JB;
             po[u] = O[min(jo, n - 1) + (size_t)min(c0 + cl, l - 1) * ldo];
         }
-        const double pn = nj[min(j0 + (tid % JB), n - 1)];
-        __syncthreads();  // the previous chunk's readers are done
+        pn = nj[min(j0 + (tid % JB), n - 1)];
+    };
+    auto store = [&](int j0, int b) {
+        const int jo = j0 + (tid % JB);
 #pragma unroll
         for (int u = 0; u < NXL; ++u) {
             const int e = tid + 256 * u, jj = e / FD, f = e % FD;
-            sX[f][jj] = (j0 + jj < n && f < d) ? px[u] : 0.0;
+            sX[b][jj * XP + f] = (j0 + jj < n && f < d) ? px[u] : 0.0;
         }
+        double* so = (double*)&sO[b][0][0][0];
 #pragma unroll
         for (int u = 0; u < NOL; ++u) {
             const int e = tid + 256 * u, jj = e % JB, cl = e / JB;
-            sO[jj >> 2][cl >> 4][(jj & 3) * 16 + (cl & 15)] =
+            const int r = jj >> 2, ln = (jj & 3) * 16 + (cl & 15);
+            so[(((cl >> 4) * 2 + (r >> 1)) * 64 + ln) * 2 + (r & 1)] =
                 (jo < n && c0 + cl < l) ? po[u] * yv : 0.0;
         }
-        if (tid < JB) sN[tid] = (j0 + tid < n) ? pn : 0.0;
-        __syncthreads();
+        if (tid < JB) sN[b][tid] = (j0 + tid < n) ? pn : 0.0;
+    };
 
-        dh_d4 s = dh_d4{0, 0, 0, 0};
+    load(0);
+    store(0, 0);
+    __syncthreads();
+    int b = 0;
+    for (int j0 = 0; j0 < n; j0 += JB) {
+        const bool more = j0 + JB < n;  // uniform
+        if (more) load(j0 + JB);
+        dh_d4 s0 = dh_d4{0, 0, 0, 0}, s1 = dh_d4{0, 0, 0, 0};
+        const double* xrow = &sX[b][li * XP + g * DK];
 #pragma unroll
-        for (int kk = 0; kk < DK; ++kk)
-            s = __builtin_amdgcn_mfma_f64_16x16x4f64(sX[4 * kk + g][li], xr[kk], s, 0, 0, 0);
+        for (int kk = 0; kk < DK; kk += 2) {
+            const d2v a2 = *(const d2v*)(xrow + kk);
+            s0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a2[0], xr[kk], s0, 0, 0, 0);
+            s1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a2[1], xr[kk + 1], s1, 0, 0, 0);
+        }
         double kv[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            double d2 = nrow + sN[g + 4 * r] - 2.0 * s[r];
+            double d2 = nrow + sN[b][g + 4 * r] - 2.0 * (s0[r] + s1[r]);
             d2 = d2 > 0.0 ? d2 : 0.0;
             kv[r] = exp(-gamma * d2);
         }
 #pragma unroll
-        for (int cb = 0; cb < NCB; ++cb)
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(kv[r], sO[r][cb][lane], acc[cb], 0,
-                                                               0, 0);
+        for (int cb = 0; cb < NCB; ++cb) {
+            const d2v o01 = sO[b][cb][0][lane], o23 = sO[b][cb][1][lane];
+            acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(kv[0], o01[0], acc[cb], 0, 0, 0);
+            acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(kv[1], o01[1], acc[cb], 0, 0, 0);
+            acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(kv[2], o23[0], acc[cb], 0, 0, 0);
+            acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(kv[3], o23[1], acc[cb], 0, 0, 0);
+        }
+        if (more) store(j0 + JB, b ^ 1);
+        __syncthreads();
+        b ^= 1;
     }
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb)
@@ -247,10 +279,12 @@ static void launch_project_f64(dh_handle_t h, dim3 grid, int m, int n, int d, in
                                const double* ni, const double* nj, const double* yi,
                                const double* yj, double gamma, const double* O, int ldo,
                                double beta, double* Y, int ldy) {
-    dh_timing_begin(h);
-    rbf_project_f64_kernel<DK, NCB><<<grid, 256, 0, h->stream>>>(
-        m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, O, ldo, beta, Y, ldy);
-    dh_timing_end(h);
+    if constexpr (DK < 64 || NCB <= 12) {  // d > 128 with 17 blocks is never selected (spills)
+        dh_timing_begin(h);
+        rbf_project_f64_kernel<DK, NCB><<<grid, 256, 0, h->stream>>>(
+            m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, O, ldo, beta, Y, ldy);
+        dh_timing_end(h);
+    }
 }
 
 template <int DK>
@@ -297,7 +331,8 @@ int dh_project_f64_impl(dh_handle_t h, int m, int n, int d, int l, const double*
                         int ldo, double beta, double* Y, int ldy) {
     if (m == 0 || l == 0) return 0;
     int ncb_total = dh_cdiv(l, 16);
-    int groups = dh_cdiv(ncb_total, 17);
+    // Above 128 features the 17-block tile spills its accumulators: cap at 12.
+    int groups = dh_cdiv(ncb_total, d > 128 ? 12 : 17);
     int ncb = ncb_instance(dh_cdiv(ncb_total, groups));
     groups = dh_cdiv(ncb_total, ncb);
     dim3 grid(dh_cdiv(m, 64), groups);
