@@ -29,6 +29,7 @@ enum dh_slot {
     DH_WS_TRSM_X,      // solution blocks of the block-sweep triangular solve
     DH_WS_EIG,         // Jacobi SVD / Nystrom work matrices
     DH_WS_SVM,         // vectors and partial sums of the barrier SVM solver
+    DH_WS_KBLOCK,      // materialised K row block of the d > 256 FP64 pass
     DH_WS_COUNT
 };
 

@@ -353,6 +353,9 @@ extern "C" int dh_lra(dh_handle_t h, int prec, int n, int d, int l, int q, const
     if (ldq < max(1, n)) return dh_fail(-14, "%s: ldq < n", name);
     if (ldw < max(1, n)) return dh_fail(-16, "%s: ldw < n", name);
     if (n == 0 || l == 0) return 0;
+    // The low-precision kernels hold at most 256 features: above that every
+    // precision runs the FP64 passes (materialised K row blocks).
+    if (d > 256) prec = DH_PREC_F64;
     int st;
     if (prec == DH_PREC_F64)
         st = lra_f64(h, n, d, l, q, X, ldx, y, gamma, Omega, ldo, Q, ldq, W, ldw);

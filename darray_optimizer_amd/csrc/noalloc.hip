@@ -66,6 +66,8 @@ extern "C" int dh_rbf_project_f64_noalloc(dh_handle_t h, int m, int n, int d, in
     if (ldxj < max(1, d)) return dh_fail(-9, "%s: ldxj < d", name);
     if (ldo < max(1, n)) return dh_fail(-14, "%s: ldo < n", name);
     if (ldy < max(1, m)) return dh_fail(-17, "%s: ldy < m", name);
+    if (d > 256)
+        return dh_fail(DH_ERR_UNSUPPORTED, "%s: d = %d > 256 needs a K block workspace", name, d);
     if ((long)m + n > DH_NOALLOC_MAX_POINTS)
         return dh_fail(DH_ERR_UNSUPPORTED,
                        "%s: m + n = %ld exceeds the %d points of the static norm buffer", name,

@@ -317,6 +317,36 @@ static int dispatch_ncb(dh_handle_t h, int ncb, dim3 grid, int m, int n, int d, 
     return 0;
 }
 
+// Above 256 features a wave's rows of Xi no longer fit in registers.  K is
+// then materialised in row blocks of at most 2^27 entries (1 GiB), each by the
+// distance-GEMM tile kernel (FP64 MFMA, exp epilogue), and multiplied into
+// Omega by the FP64 GEMM: Y_blk = K_blk Omega + beta Y_blk.  The same split as
+// rbf_kernel's explicit tiles (dkernel.cu:144-178), with the product fused in.
+static int project_f64_blocked(dh_handle_t h, int m, int n, int d, int l, const double* Xi,
+                               int ldxi, const double* Xj, int ldxj, const double* ni,
+                               const double* nj, const double* yi, const double* yj, double gamma,
+                               const double* O, int ldo, double beta, double* Y, int ldy) {
+    if (n == 0)  // K has no columns: Y = beta Y
+        return dh_gemm_impl<double>(h, false, false, m, l, 0, 1.0, O, 1, O, ldo, beta, Y, ldy, 0);
+    const long cap = 1L << 27;
+    int mb = (int)std::min<long>(m, std::max<long>(64, cap / n / 64 * 64));
+    double* Kb = (double*)dh_ws(h, DH_WS_KBLOCK, sizeof(double) * (size_t)mb * n);
+    if (!Kb) return DH_ERR_ALLOC;
+    dh_timing_begin(h);
+    for (int i0 = 0; i0 < m; i0 += mb) {
+        const int ib = std::min(mb, m - i0);
+        dim3 grid(dh_cdiv(ib, 64), dh_cdiv(n, 64));
+        rbf_tile_kernel<double, 64, 64><<<grid, 256, 0, h->stream>>>(
+            ib, n, d, Xi + (size_t)i0 * ldxi, ldxi, Xj, ldxj, ni + i0, nj, yi ? yi + i0 : nullptr,
+            yj, gamma, Kb, ib);
+        DH_CHECK_LAUNCH();
+        DH_CHECK_STATUS(dh_gemm_impl<double>(h, false, false, ib, l, n, 1.0, Kb, ib, O, ldo, beta,
+                                             Y + i0, ldy, 0));
+    }
+    dh_timing_end(h);
+    return 0;
+}
+
 // Smallest instantiated column-block count >= want.
 static int ncb_instance(int want) {
     static const int inst[] = {1, 2, 3, 4, 5, 6, 8, 9, 12, 17};
@@ -348,7 +378,8 @@ int dh_project_f64_impl(dh_handle_t h, int m, int n, int d, int l, const double*
     if (d <= 256)
         return dispatch_ncb<64>(h, ncb, grid, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj,
                                 gamma, O, ldo, beta, Y, ldy);
-    return dh_fail(DH_ERR_UNSUPPORTED, "rbf_project_f64: d = %d > 256", d);
+    return project_f64_blocked(h, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, O, ldo,
+                               beta, Y, ldy);
 }
 
 extern "C" int dh_rbf_project_f64(dh_handle_t h, int m, int n, int d, int l, const double* Xi,

@@ -100,8 +100,11 @@ int dh_rbf_tile_f32(dh_handle_t h, int m, int n, int d, const float* Xi, int ldx
                     float* K, int ldk);
 
 /* Fused Y[m x l] = K(Xi,Xj) * O[n x l] + beta*Y in FP64 (MFMA f64 for the distance
- * and the projection, FP64 exp).  K is never stored.  Replaces the rbf_kernel +
- * replicate + tc_gemm sequence of DArray::LRA (dkernel.h:33-42). */
+ * and the projection, FP64 exp).  K is never stored for d <= 256; above 256
+ * features K is materialised in row blocks of at most 2^27 entries (a 1 GiB
+ * workspace), as rbf_kernel does (dkernel.cu:144-178), and multiplied into O by
+ * the FP64 GEMM.  Replaces the rbf_kernel + replicate + tc_gemm sequence of
+ * DArray::LRA (dkernel.h:33-42). */
 int dh_rbf_project_f64(dh_handle_t h, int m, int n, int d, int l, const double* Xi, int ldxi,
                        const double* Xj, int ldxj, const double* yi, const double* yj,
                        double gamma, const double* O, int ldo, double beta, double* Y, int ldy);
@@ -112,8 +115,8 @@ int dh_rbf_project_f64(dh_handle_t h, int m, int n, int d, int l, const double* 
  * forbid hipMalloc).  Its only scratch, the squared norms of Xi and Xj, lives
  * in a static device array of DH_NOALLOC_MAX_POINTS doubles (8 MiB, held by
  * every GPU the library runs on until the process ends; dh_create looks up
- * its address once per device), so m + n <= DH_NOALLOC_MAX_POINTS, else
- * DH_ERR_UNSUPPORTED before any launch.  The call makes no hipMalloc,
+ * its address once per device), so m + n <= DH_NOALLOC_MAX_POINTS and
+ * d <= 256 (no K block), else DH_ERR_UNSUPPORTED before any launch.  The call makes no hipMalloc,
  * hipFree or other allocation.  Calls that overlap on one GPU (any handle,
  * any stream) race on the array. */
 #define DH_NOALLOC_MAX_POINTS (1 << 20)
@@ -241,7 +244,8 @@ int dh_cholqr2_f64(dh_handle_t h, int m, int n, double* Y, int ldy, double* R, i
  * where CholeskyQR cannot orthonormalise it, the deficient columns are
  * replaced by pseudo-random ones, so Q always has l orthonormal columns
  * spanning range(Y) and more (as the reference's Householder QR would).
- * Synchronises. */
+ * Above 256 features every `prec` runs the FP64 passes (the low-precision
+ * kernels hold d <= 256).  Synchronises. */
 int dh_lra(dh_handle_t h, int prec, int n, int d, int l, int q, const double* X, int ldx,
            const double* y, double gamma, const double* Omega, int ldo, double* Q, int ldq,
            double* W, int ldw);

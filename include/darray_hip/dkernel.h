@@ -29,8 +29,8 @@ namespace DArray {
 // FP32 accumulation (DH_PREC_F16: points centred, scaled by a power of two and
 // rounded once; the mixed mode whose approximation error stays within 1.1x of
 // FP64); BF16 (DH_PREC_BF16, same kernels with 3 fewer mantissa bits); FP8
-// (DH_PREC_FP8: e4m3 distance, BF16 projection).  The 16-bit and FP8 modes
-// take d <= 256.
+// (DH_PREC_FP8: e4m3 distance, BF16 projection).  Above 256 features every
+// mode runs the FP64 passes (the 16-bit and FP8 kernels hold d <= 256).
 enum class Precision {
     FP64 = DH_PREC_F64,
     Mixed = DH_PREC_F16,

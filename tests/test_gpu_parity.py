@@ -80,7 +80,8 @@ def test_rbf_tile_golden(dev):
 # --------------------------------------------------------- fused projection
 
 PROJ_SHAPES = [(1, 1, 1, 1), (64, 64, 64, 16), (100, 77, 13, 9), (129, 65, 33, 40),
-               (300, 257, 130, 138), (200, 190, 200, 300), (96, 500, 128, 272)]
+               (300, 257, 130, 138), (200, 190, 200, 300), (96, 500, 128, 272),
+               (150, 300, 257, 40), (70, 130, 600, 17)]  # d > 256: materialised K blocks
 
 
 @pytest.mark.parametrize("m,n,d,l", PROJ_SHAPES)
@@ -366,6 +367,29 @@ def test_lra_f64_vs_oracle(dev, labels, q):
     eo, ko = orc.lra_error(X, 1.0 / d, Qo, Wo, y)
     assert abs(k2 - ko) / ko < 1e-12
     assert abs(np.sqrt(e2 / k2) - np.sqrt(eo / ko)) <= 1e-10 + 1e-6 * np.sqrt(eo / ko)
+
+
+def test_lra_above_256_features(dev):
+    """Above 256 features the passes materialise K in row blocks (FP64); every
+    precision runs them, so the low-precision requests return the FP64 result
+    bit for bit."""
+    import torch
+    n, d, l = 700, 300, 24
+    X = dh.fill_uniform(3, d, n).T.copy()
+    Om = dh.fill_normal(4, n, l)
+    y = np.where(np.arange(n) % 4 == 0, -1.0, 1.0)
+    out = {}
+    for prec in (dh.DH_PREC_F64, dh.DH_PREC_BF16, dh.DH_PREC_F16, dh.DH_PREC_FP8):
+        Q = empty_mat(n, l, torch.float64)
+        W = empty_mat(n, l, torch.float64)
+        assert dev.lra(dev_pts(X), dev_mat(Om), 1.0 / d, 1, Q, W, dev_vec(y), prec=prec) == 0
+        out[prec] = (dh.from_dev(Q), dh.from_dev(W))
+    Qo, Wo, info = orc.lra(X, Om, 1.0 / d, 1, y, orth="householder")
+    assert info == 0
+    Qg, Wg = out[dh.DH_PREC_F64]
+    assert rel(Qg @ Wg.T, Qo @ Wo.T) < 1e-10
+    for prec in (dh.DH_PREC_BF16, dh.DH_PREC_F16, dh.DH_PREC_FP8):
+        assert np.array_equal(out[prec][0], Qg) and np.array_equal(out[prec][1], Wg)
 
 
 def test_lra_golden(dev):

@@ -73,5 +73,11 @@ def test_noalloc_refuses_sizes_beyond_the_static_array():
         assert st == _capi.DH_ERR_UNSUPPORTED
         assert "static norm buffer" in _capi.last_error()
         assert torch.count_nonzero(Y) == 0
+        # d > 256 would need a K block workspace: refused before any launch
+        st = _capi.lib().dh_rbf_project_f64_noalloc(
+            h.h, 8, 1, 300, 1, _p(X), 300, _p(X), 300, None, None, 1.0, _p(X), 1, 0.0, _p(Y), 8)
+        assert st == _capi.DH_ERR_UNSUPPORTED
+        assert torch.count_nonzero(Y) == 0
+        assert h.workspace_bytes() == 0
     finally:
         h.close()
