@@ -11,9 +11,12 @@
 // Per CG iteration Z is read twice (Z^T p by row chunks and column groups,
 // then Z v by rows): 2 n k 8 bytes, HBM-bound.  All reductions are
 // per-workgroup partial sums combined in a fixed order, so results are
-// deterministic.  Scalars of the CG recurrence stay on the device; the host
-// reads the residual norm once per iteration and the line-search values once
-// per Newton step.
+// deterministic.  Scalars of the CG recurrence stay on the device, and so
+// does the stopping test: the last kernel of an iteration clears an active
+// flag that the kernels of later iterations check first, so the host queues
+// iterations in batches and reads the flag once per batch (the iterate is the
+// one an iteration-by-iteration loop would stop at).  The line-search values
+// are read once per Newton step.
 //
 // Deviations from the reference (bug fixes), as the oracle states them: the
 // CG right-hand side is -grad (the reference solves with b = e_1,
@@ -67,9 +70,15 @@ __device__ double fixed_sum(int count, const double* __restrict__ part, double* 
 }
 
 // part[c * nchunks + chunk] = sum over the chunk's rows of Z[r, c] * x[r].
+// Kernels of the CG iteration take `ctl` (the loop's control words, ctl[0] =
+// active flag) and do nothing once the loop has stopped; NULL = always run.
+__device__ __forceinline__ bool cg_stopped(const double* ctl) { return ctl && ctl[0] == 0.0; }
+
 __global__ __launch_bounds__(256) void ztx_partial_kernel(int n, int k, const double* __restrict__ Z,
                                                           int ldz, const double* __restrict__ x,
-                                                          double* __restrict__ part) {
+                                                          double* __restrict__ part,
+                                                          const double* __restrict__ ctl) {
+    if (cg_stopped(ctl)) return;
     const int chunk = blockIdx.x, nchunks = gridDim.x;
     const int r0 = chunk * kRows;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -98,7 +107,9 @@ __global__ __launch_bounds__(256) void ztx_partial_kernel(int n, int k, const do
 // out[c] = sum over chunks of part[c * nchunks + chunk]; one wave per column.
 __global__ __launch_bounds__(256) void reduce_cols_kernel(int nchunks, int k,
                                                           const double* __restrict__ part,
-                                                          double* __restrict__ out) {
+                                                          double* __restrict__ out,
+                                                          const double* __restrict__ ctl) {
+    if (cg_stopped(ctl)) return;
     const int c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (c >= k) return;
     const double* p = part + (size_t)c * nchunks;
@@ -215,7 +226,9 @@ __global__ __launch_bounds__(256) void cg_apply_kernel(int n, int k, const doubl
                                                        const double* __restrict__ D,
                                                        const double* __restrict__ p,
                                                        double* __restrict__ q,
-                                                       double* __restrict__ part) {
+                                                       double* __restrict__ part,
+                                                       const double* __restrict__ ctl) {
+    if (cg_stopped(ctl)) return;
     __shared__ double su[kMaxK];
     __shared__ double sh[4];
     for (int c = threadIdx.x; c < k; c += 256) su[c] = v[c];
@@ -243,7 +256,9 @@ __global__ __launch_bounds__(256) void cg_update_kernel(int n, int nblk,
                                                         double* __restrict__ s,
                                                         double* __restrict__ r,
                                                         double* __restrict__ z,
-                                                        double* __restrict__ part2) {
+                                                        double* __restrict__ part2,
+                                                        const double* __restrict__ ctl) {
+    if (cg_stopped(ctl)) return;
     __shared__ double sh[4];
     const double pq = fixed_sum(nblk, part_pq, sh);
     const double alpha = sc[0] / pq;
@@ -274,7 +289,9 @@ __global__ __launch_bounds__(256) void cg_direction_kernel(int n, int nblk,
                                                            const double* __restrict__ sc,
                                                            const double* __restrict__ z,
                                                            double* __restrict__ p,
-                                                           double* __restrict__ sc_next) {
+                                                           double* __restrict__ sc_next,
+                                                           const double* __restrict__ ctl) {
+    if (cg_stopped(ctl)) return;
     __shared__ double sh[4];
     double v0 = 0.0, v1 = 0.0;
     for (int b = threadIdx.x; b < nblk; b += 256) {
@@ -301,11 +318,26 @@ __global__ __launch_bounds__(256) void cg_direction_kernel(int n, int nblk,
     }
 }
 
-// sc[0..1] = sc_next[0..1] (the next iteration's r.z and the residual norm).
-__global__ void cg_commit_kernel(double* __restrict__ sc, const double* __restrict__ sc_next) {
-    if (threadIdx.x == 0) {
+// End of an iteration: sc[0..1] = sc_next[0..1] (the next iteration's r.z and
+// the residual norm), one more iteration counted in ctl[1], and the stopping
+// test of the loop, sqrt(r.r) <= tol or max_iter iterations, into ctl[0];
+// ctl[2] = r.r for the host.
+__global__ void cg_commit_kernel(double* __restrict__ sc, const double* __restrict__ sc_next,
+                                 double* __restrict__ ctl, double tol, int max_iter) {
+    if (threadIdx.x == 0 && ctl[0] != 0.0) {
         sc[0] = sc_next[0];
         sc[1] = sc_next[1];
+        ctl[1] += 1.0;
+        ctl[2] = sc[1];
+        ctl[0] = (sqrt(sc[1]) > tol && ctl[1] < (double)max_iter) ? 1.0 : 0.0;
+    }
+}
+
+// Loop start: active, no iterations yet.
+__global__ void cg_start_kernel(double* __restrict__ ctl) {
+    if (threadIdx.x == 0) {
+        ctl[0] = 1.0;
+        ctl[1] = 0.0;
     }
 }
 
@@ -405,6 +437,7 @@ struct Work {
     double *part, *part2;       // per-workgroup partials
     double *ztpart;             // nchunks x k partials of Z^T x
     double *sc, *sc_next, *ls;  // device scalars
+    double* ctl;                // CG loop control: active flag, iterations, r.r
 };
 
 int grab(dh_handle_t h, int n, int k, Work& w) {
@@ -415,7 +448,7 @@ int grab(dh_handle_t h, int n, int k, Work& w) {
     w.ngrp = dh_cdiv(k, kColGrp);
     const size_t nn = (size_t)std::max(n, 1);
     const size_t doubles = 10 * nn + 2 * (size_t)k + (size_t)w.nblk * (kLsW + 4) +
-                           2 * (size_t)w.nblk + (size_t)w.nchunks * k + 64 + kLsW + 8;
+                           2 * (size_t)w.nblk + (size_t)w.nchunks * k + 80 + kLsW + 8;
     double* base = (double*)dh_ws(h, DH_WS_SVM, sizeof(double) * doubles);
     if (!base) return DH_ERR_ALLOC;
     double* x = base;
@@ -442,15 +475,19 @@ int grab(dh_handle_t h, int n, int k, Work& w) {
     w.sc = take(16);
     w.sc_next = take(16);
     w.ls = take(kLsW + 8 + 32);
+    w.ctl = take(16);
     return 0;
 }
 
-// out (k) = Z^T x.
-int zt(dh_handle_t h, const Work& w, const double* Z, int ldz, const double* x, double* out) {
+// out (k) = Z^T x (skipped once the CG loop controlled by ctl has stopped;
+// ctl = NULL: always).
+int zt(dh_handle_t h, const Work& w, const double* Z, int ldz, const double* x, double* out,
+       const double* ctl = nullptr) {
     ztx_partial_kernel<<<dim3(w.nchunks, w.ngrp), 256, 0, h->stream>>>(w.n, w.k, Z, ldz, x,
-                                                                       w.ztpart);
+                                                                       w.ztpart, ctl);
     DH_CHECK_LAUNCH();
-    reduce_cols_kernel<<<dh_cdiv(w.k, 4), 256, 0, h->stream>>>(w.nchunks, w.k, w.ztpart, out);
+    reduce_cols_kernel<<<dh_cdiv(w.k, 4), 256, 0, h->stream>>>(w.nchunks, w.k, w.ztpart, out,
+                                                               ctl);
     DH_CHECK_LAUNCH();
     return 0;
 }
@@ -464,28 +501,44 @@ int read_scalars(dh_handle_t h, const double* dev, double* host, int count) {
     return 0;
 }
 
+// Iterations queued between two reads of the CG loop's control words.
+constexpr int kCgBatch = 8;
+
 // Preconditioned CG from the state left by newton_setup_kernel (or
-// pcg_setup): iterates until sqrt(r.r) <= tol or max_iter.
+// pcg_setup): iterates until sqrt(r.r) <= tol or max_iter.  The test runs on
+// the device at the end of every iteration (cg_commit_kernel); iterations
+// queued after it fails do nothing.
 int cg_loop(dh_handle_t h, const Work& w, const double* Z, int ldz, double tol, int max_iter,
             int* iters, double* rnorm) {
     double rr = 0.0;
     DH_CHECK_STATUS(read_scalars(h, w.sc + 1, &rr, 1));
     int it = 0;
-    while (std::sqrt(rr) > tol && it < max_iter) {
-        DH_CHECK_STATUS(zt(h, w, Z, ldz, w.p, w.v));
-        cg_apply_kernel<<<w.nblk, 256, 0, h->stream>>>(w.n, w.k, Z, ldz, w.v, w.D, w.p, w.q,
-                                                        w.part);
+    if (std::sqrt(rr) > tol && it < max_iter) {
+        cg_start_kernel<<<1, 64, 0, h->stream>>>(w.ctl);
         DH_CHECK_LAUNCH();
-        cg_update_kernel<<<w.nblk, 256, 0, h->stream>>>(w.n, w.nblk, w.part, w.sc, w.p, w.q, w.Pc,
-                                                         w.s, w.r, w.z, w.part2);
-        DH_CHECK_LAUNCH();
-        cg_direction_kernel<<<w.nblk, 256, 0, h->stream>>>(w.n, w.nblk, w.part2, w.sc, w.z, w.p,
-                                                            w.sc_next);
-        DH_CHECK_LAUNCH();
-        cg_commit_kernel<<<1, 64, 0, h->stream>>>(w.sc, w.sc_next);
-        DH_CHECK_LAUNCH();
-        DH_CHECK_STATUS(read_scalars(h, w.sc + 1, &rr, 1));
-        ++it;
+        for (;;) {
+            const int batch = std::min(kCgBatch, max_iter - it);
+            for (int b = 0; b < batch; ++b) {
+                DH_CHECK_STATUS(zt(h, w, Z, ldz, w.p, w.v, w.ctl));
+                cg_apply_kernel<<<w.nblk, 256, 0, h->stream>>>(w.n, w.k, Z, ldz, w.v, w.D, w.p,
+                                                                w.q, w.part, w.ctl);
+                DH_CHECK_LAUNCH();
+                cg_update_kernel<<<w.nblk, 256, 0, h->stream>>>(w.n, w.nblk, w.part, w.sc, w.p,
+                                                                 w.q, w.Pc, w.s, w.r, w.z,
+                                                                 w.part2, w.ctl);
+                DH_CHECK_LAUNCH();
+                cg_direction_kernel<<<w.nblk, 256, 0, h->stream>>>(w.n, w.nblk, w.part2, w.sc,
+                                                                    w.z, w.p, w.sc_next, w.ctl);
+                DH_CHECK_LAUNCH();
+                cg_commit_kernel<<<1, 64, 0, h->stream>>>(w.sc, w.sc_next, w.ctl, tol, max_iter);
+                DH_CHECK_LAUNCH();
+            }
+            double c[3];
+            DH_CHECK_STATUS(read_scalars(h, w.ctl, c, 3));
+            it = (int)c[1];
+            rr = c[2];
+            if (c[0] == 0.0) break;
+        }
     }
     if (iters) *iters = it;
     if (rnorm) *rnorm = std::sqrt(rr);
