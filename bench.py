@@ -8,9 +8,10 @@ resident in HBM.  GFLOP/s counts F_MFMA + F_orth:
 The headline is the mixed path whose error matches FP64 (||K-QB||/||K||
 within 1.1x): FP16 K-passes on power-of-two scaled coordinates with FP32
 accumulation and FP64 CholeskyQR2.  Besides it the line carries the FP64
-solve, the BF16 K-passes (faster, less accurate), the 16384 core solve, the
-batched workload (96 problems, one graph), the largest workload (n = 262144,
-d = 256, k = 512, FP8 distance) and the CPU baseline.
+solve, the BF16 K-passes (faster, less accurate), C0's workload (n = 8192) on
+the GPU, the 16384 core solve, the batched workload (96 problems, one graph),
+the largest workload (n = 262144, d = 256, k = 512, FP8 distance) and the CPU
+baseline.
 Prints ONE JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
 """
 from __future__ import annotations
@@ -68,6 +69,37 @@ def host_threads() -> int:
     except (AttributeError, OSError):
         avail = os.cpu_count() or 1
     return max(1, min(16, avail))
+
+
+def c0_on_gpu(h, with_err):
+    """C0's workload on the GPU, the same inputs as the CPU baseline's full
+    solve: n = 8192 points uniform in [0,1)^64 (set_uniform seed 0), Omega
+    from set_normal_seed(1), l = 138, q = 1, gamma = 1/d; FP64 and FP16 passes."""
+    import torch
+    import darray_optimizer_amd as dh
+    n0, d0, l0, q0 = 8192, 64, 138, 1
+    X0 = torch.from_numpy(np.ascontiguousarray(dh.fill_uniform(0, d0, n0).T)).cuda()
+    Om0 = dh.to_dev(np.asfortranarray(dh.fill_normal(1, n0, l0)))
+    Q = torch.empty((l0, n0), dtype=torch.float64, device="cuda")
+    W = torch.empty_like(Q)
+    fm0, fo0 = lra_flops(n0, d0, l0, q0)
+    out = {"n": n0, "d": d0, "l": l0, "q": q0}
+    for name, prec in (("fp64", dh.DH_PREC_F64), ("f16", dh.DH_PREC_F16)):
+        for _ in range(2):
+            assert h.lra(X0, Om0, 1.0 / d0, q0, Q, W, prec=prec) == 0
+        torch.cuda.synchronize()
+        reps = 10
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            assert h.lra(X0, Om0, 1.0 / d0, q0, Q, W, prec=prec) == 0
+        torch.cuda.synchronize()
+        s = (time.perf_counter() - t0) / reps
+        r = {"s_per_solve": s, "gflops": (fm0 + fo0) / s / 1e9}
+        if with_err:
+            e2, k2 = h.lra_error(X0, 1.0 / d0, Q, W)
+            r["rel_err"] = float(np.sqrt(max(e2, 0.0) / k2))
+        out[name] = r
+    return out
 
 
 def cpu_baseline(X1, Om1, gamma1, rows):
@@ -460,6 +492,7 @@ def main(argv=None) -> int:
         line["err_ratio_vs_fp64"] = mixed["rel_err"] / fp64["rel_err"]
         bf16["err_ratio_vs_fp64"] = bf16["rel_err"] / fp64["rel_err"]
     bf16["speedup_vs_fp64"] = fp64["s_per_solve"] / bf16["s_per_solve"]
+    line["c0_gpu"] = c0_on_gpu(h, not a.no_error)
     if a.consumer:
         line["consumer"] = consumer_steps(h, Xd, X, gamma, n, d, l, a.k, q)
     if a.batched > 0:

@@ -17,10 +17,16 @@
 // Breakdown: a problem whose Gram matrix is numerically singular keeps its
 // flag in the device info words; after the graph the host redoes such
 // problems alone with dh_lra, which falls back to shifted CholeskyQR3.
+#include <sys/stat.h>
+
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
+#include <string>
+#include <utility>
 #include <vector>
 
 #include "batch.h"
@@ -130,7 +136,83 @@ struct Plan {
 
 struct TuneTable {
     std::map<uint64_t, int> choice;
+    std::string arch;  // device tag of the file cache ("" until first needed)
 };
+
+// Process-wide file cache of autotuner decisions, shared by all handles:
+// one line "<device tag> <key, hex> <choice>" per decision, later lines
+// overriding earlier ones.  The device tag is the architecture name and the
+// CU count, so a file shared between machines keeps their choices apart.
+// Path: DH_TUNE_CACHE if set (empty disables the file), else
+// $XDG_CACHE_HOME/darray_hip/tune.txt, else $HOME/.cache/darray_hip/tune.txt.
+// A file that cannot be read or written leaves the in-memory tables working.
+struct TuneFile {
+    std::mutex mu;
+    bool loaded = false;
+    std::string path;
+    std::map<std::pair<std::string, uint64_t>, int> entries;
+};
+
+TuneFile& tune_file() {
+    static TuneFile f;
+    return f;
+}
+
+std::string tune_cache_path() {
+    if (const char* e = std::getenv("DH_TUNE_CACHE")) return std::string(e);
+    std::string base;
+    if (const char* x = std::getenv("XDG_CACHE_HOME"); x && *x) {
+        base = x;
+    } else if (const char* hm = std::getenv("HOME"); hm && *hm) {
+        base = std::string(hm) + "/.cache";
+    } else {
+        return "";
+    }
+    return base + "/darray_hip/tune.txt";
+}
+
+void tune_file_load(TuneFile& f) {  // f.mu held
+    if (f.loaded) return;
+    f.loaded = true;
+    f.path = tune_cache_path();
+    if (f.path.empty()) return;
+    FILE* fp = std::fopen(f.path.c_str(), "r");
+    if (!fp) return;
+    char line[512], arch[160];
+    unsigned long long key = 0;
+    int choice = 0;
+    while (std::fgets(line, sizeof line, fp))
+        if (line[0] != '#' && std::sscanf(line, "%159s %llx %d", arch, &key, &choice) == 3)
+            f.entries[{std::string(arch), (uint64_t)key}] = choice;
+    std::fclose(fp);
+}
+
+void tune_file_append(TuneFile& f, const std::string& arch, uint64_t key, int choice) {
+    if (f.path.empty()) return;
+    // Parent directories, one level at a time (errors surface at fopen).
+    for (size_t pos = f.path.find('/', 1); pos != std::string::npos;
+         pos = f.path.find('/', pos + 1))
+        (void)mkdir(f.path.substr(0, pos).c_str(), 0755);
+    FILE* fp = std::fopen(f.path.c_str(), "a");
+    if (!fp) return;
+    std::fprintf(fp, "%s %llx %d\n", arch.c_str(), (unsigned long long)key, choice);
+    std::fclose(fp);
+}
+
+const std::string& tune_arch(dh_handle_t h) {
+    auto* t = static_cast<TuneTable*>(h->tune);
+    if (t->arch.empty()) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, h->device) == hipSuccess) {
+            t->arch = std::string(p.gcnArchName) + "/" + std::to_string(p.multiProcessorCount);
+            for (char& c : t->arch)
+                if (c == ' ') c = '_';
+        } else {
+            t->arch = "unknown";
+        }
+    }
+    return t->arch;
+}
 
 // Bump allocator over one device block (256-byte aligned pieces).
 struct Carve {
@@ -187,15 +269,31 @@ int dh_forced_schedule() {
 }
 
 int dh_tune_lookup(dh_handle_t h, uint64_t key) {
-    if (!h->tune) return -1;
+    if (!h->tune) h->tune = new TuneTable();
     auto& m = static_cast<TuneTable*>(h->tune)->choice;
     auto it = m.find(key);
-    return it == m.end() ? -1 : it->second;
+    if (it != m.end()) return it->second;
+    const std::string& arch = tune_arch(h);
+    TuneFile& f = tune_file();
+    std::lock_guard<std::mutex> lock(f.mu);
+    tune_file_load(f);
+    auto fi = f.entries.find({arch, key});
+    if (fi == f.entries.end()) return -1;
+    m[key] = fi->second;
+    return fi->second;
 }
 
 void dh_tune_store(dh_handle_t h, uint64_t key, int choice) {
     if (!h->tune) h->tune = new TuneTable();
     static_cast<TuneTable*>(h->tune)->choice[key] = choice;
+    const std::string& arch = tune_arch(h);
+    TuneFile& f = tune_file();
+    std::lock_guard<std::mutex> lock(f.mu);
+    tune_file_load(f);
+    auto it = f.entries.find({arch, key});
+    if (it != f.entries.end() && it->second == choice) return;  // already on file
+    f.entries[{arch, key}] = choice;
+    tune_file_append(f, arch, key, choice);
 }
 
 void dh_tune_release(dh_handle_t h) {

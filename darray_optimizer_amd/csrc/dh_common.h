@@ -30,6 +30,7 @@ enum dh_slot {
     DH_WS_EIG,         // Jacobi SVD / Nystrom work matrices
     DH_WS_SVM,         // vectors and partial sums of the barrier SVM solver
     DH_WS_KBLOCK,      // materialised K row block of the d > 256 FP64 pass
+    DH_WS_SPLITN,      // partial products of a K-pass whose j range is split
     DH_WS_COUNT
 };
 
@@ -115,6 +116,17 @@ void* dh_ws(dh_handle_t h, dh_slot slot, size_t bytes);
 static inline bool dh_is_t(char c) { return c == 'T' || c == 't' || c == 'C' || c == 'c'; }
 static inline bool dh_is_n(char c) { return c == 'N' || c == 'n'; }
 static inline int dh_cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+// Number of parts the j range (nchunks chunks) of a fused K-pass is split
+// into: while the grid of `workgroups` has at most half a workgroup per CU,
+// double the parts (up to 8, each at least 16 chunks).  Decided from the
+// shape alone, so a shape always sums its partial products the same way.
+// Returns the number of non-empty parts of dh_cdiv(nchunks, parts) chunks.
+static inline int dh_j_splits(int num_cu, long workgroups, int nchunks) {
+    int s = 1;
+    while (s < 8 && workgroups * s * 2 <= num_cu && nchunks >= 32 * s) s *= 2;
+    return dh_cdiv(nchunks, dh_cdiv(nchunks, s));
+}
 
 // Dense building blocks used across translation units (dense.hip).
 template <typename T>

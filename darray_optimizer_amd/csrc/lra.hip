@@ -10,7 +10,7 @@
 
 #include "dh_common.h"
 
-int dh_project_f64_impl(dh_handle_t h, int m, int n, int d, int l, const double* Xi, int ldxi,
+int dh_project_f64_split(dh_handle_t h, int m, int n, int d, int l, const double* Xi, int ldxi,
                         const double* Xj, int ldxj, const double* ni, const double* nj,
                         const double* yi, const double* yj, double gamma, const double* O,
                         int ldo, double beta, double* Y, int ldy);
@@ -320,21 +320,21 @@ static int lra_f64(dh_handle_t h, int n, int d, int l, int q, const double* X, i
     double* nrm = (double*)dh_ws(h, DH_WS_NORMS, sizeof(double) * (size_t)n);
     if (!nrm) return DH_ERR_ALLOC;
     DH_CHECK_STATUS(dh_sqnorms_f64(h, n, d, X, ldx, nrm));
-    DH_CHECK_STATUS(dh_project_f64_impl(h, n, n, d, l, X, ldx, X, ldx, nrm, nrm, y, y, gamma,
+    DH_CHECK_STATUS(dh_project_f64_split(h, n, n, d, l, X, ldx, X, ldx, nrm, nrm, y, y, gamma,
                                         Omega, ldo, 0.0, Q, ldq));
     DH_CHECK_STATUS(dh_range_orth(h, n, l, Q, ldq));
     // Power iterations alternate the basis between Q and W (no copy for even q).
     double *cur = Q, *nxt = W;
     int ldc = ldq, ldn = ldw;
     for (int it = 0; it < q; ++it) {
-        DH_CHECK_STATUS(dh_project_f64_impl(h, n, n, d, l, X, ldx, X, ldx, nrm, nrm, y, y, gamma,
+        DH_CHECK_STATUS(dh_project_f64_split(h, n, n, d, l, X, ldx, X, ldx, nrm, nrm, y, y, gamma,
                                             cur, ldc, 0.0, nxt, ldn));
         DH_CHECK_STATUS(dh_range_orth(h, n, l, nxt, ldn));
         std::swap(cur, nxt);
         std::swap(ldc, ldn);
     }
     if (cur != Q) DH_CHECK_STATUS(copy_panel(h, n, l, cur, ldc, Q, ldq));
-    return dh_project_f64_impl(h, n, n, d, l, X, ldx, X, ldx, nrm, nrm, y, y, gamma, Q, ldq, 0.0,
+    return dh_project_f64_split(h, n, n, d, l, X, ldx, X, ldx, nrm, nrm, y, y, gamma, Q, ldq, 0.0,
                                W, ldw);
 }
 

@@ -11,7 +11,7 @@ void launch_fp8(dh_handle_t h, int m, int groups, int nchunks, int l, int ncb_to
                 int ldy) {
     rbf_project_bf16_kernel<DK, NCB, 8, false, 2, true>
         <<<dim3(dh_cdiv(m, 8 * 32), groups), 512, 0, h->stream>>>(
-            m, nchunks, l, ncb_total, Xi, Xj, ai, bj, si, sj, c2, Op, yi, Y, ldy);
+            m, nchunks, l, ncb_total, Xi, Xj, ai, bj, si, sj, c2, Op, yi, Y, ldy, nchunks, 0);
 }
 
 template <int DK>

@@ -356,15 +356,35 @@ __device__ __forceinline__ void project_rows(
             }
 }
 
+// Grid: x = row block, y = column group, z = split of the j range: split z
+// takes chunks [z cps, min((z + 1) cps, nchunks)) and writes its partial
+// product to Y + z * ystride (one split: the whole range into Y).
 template <int DK, int NCB, int WAVES, bool F16, int RB, bool D8 = false>
 __global__ __launch_bounds__(WAVES * 64, 1) void rbf_project_bf16_kernel(
     int m, int nchunks, int l, int ncb_total, const bf16x8* __restrict__ Xi,
     const bf16x8* __restrict__ Xj, const float* __restrict__ ai, const float* __restrict__ bj,
     const float* __restrict__ si, const float* __restrict__ sj, float c2,
     const bf16x8* __restrict__ Op, const double* __restrict__ yi, double* __restrict__ Y,
-    int ldy) {
-    project_rows<DK, NCB, WAVES, F16, RB, D8>(blockIdx.x, blockIdx.y, m, nchunks, l, ncb_total, Xi,
-                                              Xj, ai, bj, si, sj, c2, Op, yi, Y, ldy);
+    int ldy, int cps, long ystride) {
+    const int t0 = blockIdx.z * cps;
+    const size_t p0 = (size_t)t0 * 32;
+    project_rows<DK, NCB, WAVES, F16, RB, D8>(
+        blockIdx.x, blockIdx.y, m, min(cps, nchunks - t0), l, ncb_total, Xi,
+        Xj + p0 * XSlots<DK>::XG, ai, bj + p0, si, sj ? sj + p0 : nullptr, c2,
+        Op + (size_t)t0 * ncb_total * 64, yi, Y + blockIdx.z * ystride, ldy);
+}
+
+// Y[i + c ldy] = sum over the splits z (in order) of P[z * ystride + i + c m].
+__global__ __launch_bounds__(256) void sum_splits_kernel(int m, int l, int splits,
+                                                         const double* __restrict__ P,
+                                                         long ystride, double* __restrict__ Y,
+                                                         int ldy) {
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long)m * l) return;
+    const int i = (int)(e % m), c = (int)(e / m);
+    double v = 0.0;
+    for (int z = 0; z < splits; ++z) v += P[z * ystride + e];
+    Y[i + (size_t)c * ldy] = v;
 }
 
 // Schedule used when tuning is off (DH_TUNE=0) or during graph capture.
@@ -382,12 +402,16 @@ __global__ __launch_bounds__(WAVES * 64, 1) void rbf_project_bf16_kernel(
 template <int DK, int NCB, bool F16>
 int launch_bf16(dh_handle_t h, int m, int groups, int nchunks, int l, int ncb_total,
                 const bf16x8* Xi, const bf16x8* Xj, const float* ai, const float* bj, float c2,
-                const bf16x8* Op, const double* yi, double* Y, int ldy) {
+                const bf16x8* Op, const double* yi, double* Y, int ldy, int splits,
+                long ystride) {
+    const int cps = dh_cdiv(nchunks, splits);
+    const int nz = dh_cdiv(nchunks, cps);
     auto run = [&](int sched) {
 #define DH_RUN(W, RB)                                                                        \
     rbf_project_bf16_kernel<DK, NCB, W, F16, RB>                                             \
-        <<<dim3(dh_cdiv(m, W * 16 * RB), groups), W * 64, 0, h->stream>>>(                   \
-            m, nchunks, l, ncb_total, Xi, Xj, ai, bj, nullptr, nullptr, c2, Op, yi, Y, ldy)
+        <<<dim3(dh_cdiv(m, W * 16 * RB), groups, nz), W * 64, 0, h->stream>>>(               \
+            m, nchunks, l, ncb_total, Xi, Xj, ai, bj, nullptr, nullptr, c2, Op, yi, Y, ldy, cps, \
+            ystride)
         if constexpr (DK >= 8) {
             DH_RUN(4, 2);
         } else {
@@ -399,7 +423,7 @@ int launch_bf16(dh_handle_t h, int m, int groups, int nchunks, int l, int ncb_to
 #undef DH_RUN
     };
     constexpr int ncand = DK >= 8 ? 1 : 2;
-    const uint64_t key = dh_tune_key_bf16(DK, NCB, 0, m) ^ (F16 ? (uint64_t)1 << 61 : 0);
+    const uint64_t key = dh_tune_key_bf16(DK, NCB, nz, m) ^ (F16 ? (uint64_t)1 << 61 : 0);
     int sched = ncand == 1 ? 0 : dh_tune_lookup(h, key);
     const int forced = dh_forced_schedule();
     if (ncand > 1 && forced >= 0 && forced < ncand) sched = forced;
@@ -448,10 +472,10 @@ template <int DK, bool F16>
 int dispatch_bf16_ncb(dh_handle_t h, int ncb, int groups, int m, int nchunks, int l,
                       int ncb_total, const bf16x8* Xi, const bf16x8* Xj, const float* ai,
                       const float* bj, float c2, const bf16x8* Op, const double* yi, double* Y,
-                      int ldy) {
+                      int ldy, int splits, long ystride) {
 #define DH_BF16_NCB(N) \
     case N:            \
-        return launch_bf16<DK, N, F16>(h, m, groups, nchunks, l, ncb_total, Xi, Xj, ai, bj, c2, Op, yi, Y, ldy);
+        return launch_bf16<DK, N, F16>(h, m, groups, nchunks, l, ncb_total, Xi, Xj, ai, bj, c2, Op, yi, Y, ldy, splits, ystride);
     switch (ncb) {
         DH_BF16_NCB(1)
         DH_BF16_NCB(2)
@@ -471,19 +495,24 @@ int dispatch_bf16_ncb(dh_handle_t h, int ncb, int groups, int m, int nchunks, in
 }  // namespace
 
 // Per-feature-width dispatch of one projection launch (rbf16_dk<DK>.hip):
-// BF16 or FP16 operands, column blocks ncb, `groups` column groups.
+// BF16 or FP16 operands, column blocks ncb, `groups` column groups, the j
+// range in `splits` parts whose partial products go to Y + z * ystride.
 int dh_project16_dk1(bool f16, dh_handle_t h, int ncb, int groups, int m, int nchunks, int l, int ncb_total,
                       const bf16x8* Xi, const bf16x8* Xj, const float* ai, const float* bj, float c2,
-                      const bf16x8* Op, const double* yi, double* Y, int ldy);
+                      const bf16x8* Op, const double* yi, double* Y, int ldy, int splits,
+                      long ystride);
 int dh_project16_dk2(bool f16, dh_handle_t h, int ncb, int groups, int m, int nchunks, int l, int ncb_total,
                       const bf16x8* Xi, const bf16x8* Xj, const float* ai, const float* bj, float c2,
-                      const bf16x8* Op, const double* yi, double* Y, int ldy);
+                      const bf16x8* Op, const double* yi, double* Y, int ldy, int splits,
+                      long ystride);
 int dh_project16_dk4(bool f16, dh_handle_t h, int ncb, int groups, int m, int nchunks, int l, int ncb_total,
                       const bf16x8* Xi, const bf16x8* Xj, const float* ai, const float* bj, float c2,
-                      const bf16x8* Op, const double* yi, double* Y, int ldy);
+                      const bf16x8* Op, const double* yi, double* Y, int ldy, int splits,
+                      long ystride);
 int dh_project16_dk8(bool f16, dh_handle_t h, int ncb, int groups, int m, int nchunks, int l, int ncb_total,
                       const bf16x8* Xi, const bf16x8* Xj, const float* ai, const float* bj, float c2,
-                      const bf16x8* Op, const double* yi, double* Y, int ldy);
+                      const bf16x8* Op, const double* yi, double* Y, int ldy, int splits,
+                      long ystride);
 // FP8-distance projection (rbf16_fp8.hip): points padded to dpad8 = 128 or
 // 256 one-byte features, si / sj the per-row inverse scales of Xi / Xj.
 int dh_project_fp8(dh_handle_t h, int dpad8, int ncb, int groups, int m, int nchunks, int l,

@@ -363,6 +363,8 @@ size_t packed_omega_bytes(int n, int l) {
 }
 
 // Y (m x l, FP64) = K(Xi, Xj) * Op from prepared low-precision operands.
+// Small grids split the j range over more workgroups: their partial products
+// go to a workspace and are summed in order into Y.
 template <bool F16>
 int project_16(dh_handle_t h, int m, int n, int d, int l, const LowpPoints& pi,
                  const LowpPoints& pj, float c2, const bf16x8* Op, const double* yi, double* Y,
@@ -375,20 +377,41 @@ int project_16(dh_handle_t h, int m, int n, int d, int l, const LowpPoints& pi,
     const int grid = cols.groups;
     const bf16x8* xi = (const bf16x8*)pi.xb;
     const bf16x8* xj = (const bf16x8*)pj.xb;
+    const int splits =
+        dh_j_splits(h->num_cu, (long)dh_cdiv(m, dp > 128 ? 128 : 256) * cols.groups, nchunks);
+    double* P = Y;
+    int ldp = ldy;
+    long ystride = 0;
+    if (splits > 1) {
+        ystride = (long)m * l;
+        P = (double*)dh_ws(h, DH_WS_SPLITN, sizeof(double) * (size_t)ystride * splits);
+        if (!P) return DH_ERR_ALLOC;
+        ldp = m;
+    }
+    int st;
     switch (dp) {
         case 32:
-            return dh_project16_dk1(F16, h, cols.ncb, grid, m, nchunks, l, cols.ncb_pack, xi, xj,
-                                        pi.a, pj.a, c2, Op, yi, Y, ldy);
+            st = dh_project16_dk1(F16, h, cols.ncb, grid, m, nchunks, l, cols.ncb_pack, xi, xj,
+                                  pi.a, pj.a, c2, Op, yi, P, ldp, splits, ystride);
+            break;
         case 64:
-            return dh_project16_dk2(F16, h, cols.ncb, grid, m, nchunks, l, cols.ncb_pack, xi, xj,
-                                        pi.a, pj.a, c2, Op, yi, Y, ldy);
+            st = dh_project16_dk2(F16, h, cols.ncb, grid, m, nchunks, l, cols.ncb_pack, xi, xj,
+                                  pi.a, pj.a, c2, Op, yi, P, ldp, splits, ystride);
+            break;
         case 128:
-            return dh_project16_dk4(F16, h, cols.ncb, grid, m, nchunks, l, cols.ncb_pack, xi, xj,
-                                        pi.a, pj.a, c2, Op, yi, Y, ldy);
+            st = dh_project16_dk4(F16, h, cols.ncb, grid, m, nchunks, l, cols.ncb_pack, xi, xj,
+                                  pi.a, pj.a, c2, Op, yi, P, ldp, splits, ystride);
+            break;
         default:
-            return dh_project16_dk8(F16, h, cols.ncb, grid, m, nchunks, l, cols.ncb_pack, xi, xj,
-                                        pi.a, pj.a, c2, Op, yi, Y, ldy);
+            st = dh_project16_dk8(F16, h, cols.ncb, grid, m, nchunks, l, cols.ncb_pack, xi, xj,
+                                  pi.a, pj.a, c2, Op, yi, P, ldp, splits, ystride);
+            break;
     }
+    if (st || splits == 1) return st;
+    sum_splits_kernel<<<dh_cdiv(ystride, 256), 256, 0, h->stream>>>(m, l, splits, P, ystride, Y,
+                                                                    ldy);
+    DH_CHECK_LAUNCH();
+    return 0;
 }
 
 // Carve the low-precision copies of one or two point sets out of DH_WS_LOWP_X

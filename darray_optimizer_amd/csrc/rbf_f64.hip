@@ -160,9 +160,18 @@ __global__ __launch_bounds__(256) void rbf_project_f64_kernel(
     const double* __restrict__ Xj, int ldxj, const double* __restrict__ ni,
     const double* __restrict__ nj, const double* __restrict__ yi, const double* __restrict__ yj,
     double gamma, const double* __restrict__ O, int ldo, double beta, double* __restrict__ Y,
-    int ldy) {
+    int ldy, int jps, long ystride) {
     typedef double d2v __attribute__((ext_vector_type(2)));
     constexpr int JB = 16;
+    {  // split z of the j range: points [z jps, min((z + 1) jps, n)), partial into Y + z ystride
+        const int j0 = blockIdx.z * jps;
+        n = min(jps, n - j0);
+        Xj += (size_t)j0 * ldxj;
+        nj += j0;
+        if (yj) yj += j0;
+        O += j0;
+        Y += blockIdx.z * ystride;
+    }
     constexpr int FD = DK * 4;  // padded feature count
     constexpr int XP = FD + 2;  // LDS pitch of a point (odd multiple of 16 bytes: no conflicts)
     __shared__ double sX[2][JB * XP];
@@ -278,11 +287,12 @@ static void launch_project_f64(dh_handle_t h, dim3 grid, int m, int n, int d, in
                                const double* Xi, int ldxi, const double* Xj, int ldxj,
                                const double* ni, const double* nj, const double* yi,
                                const double* yj, double gamma, const double* O, int ldo,
-                               double beta, double* Y, int ldy) {
+                               double beta, double* Y, int ldy, int jps, long ystride) {
     if constexpr (DK < 64 || NCB <= 12) {  // d > 128 with 17 blocks is never selected (spills)
         dh_timing_begin(h);
         rbf_project_f64_kernel<DK, NCB><<<grid, 256, 0, h->stream>>>(
-            m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, O, ldo, beta, Y, ldy);
+            m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, O, ldo, beta, Y, ldy, jps,
+            ystride);
         dh_timing_end(h);
     }
 }
@@ -291,11 +301,12 @@ template <int DK>
 static int dispatch_ncb(dh_handle_t h, int ncb, dim3 grid, int m, int n, int d, int l,
                         const double* Xi, int ldxi, const double* Xj, int ldxj, const double* ni,
                         const double* nj, const double* yi, const double* yj, double gamma,
-                        const double* O, int ldo, double beta, double* Y, int ldy) {
+                        const double* O, int ldo, double beta, double* Y, int ldy, int jps,
+                        long ystride) {
 #define DH_NCB(N)                                                                              \
     case N:                                                                                    \
         launch_project_f64<DK, N>(h, grid, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, \
-                                  O, ldo, beta, Y, ldy);                                       \
+                                  O, ldo, beta, Y, ldy, jps, ystride);                         \
         break;
     switch (ncb) {
         DH_NCB(1)
@@ -355,31 +366,92 @@ static int ncb_instance(int want) {
     return 17;
 }
 
+// Y (m x l) = K(Xi, Xj) O + beta Y with the fused kernel, the j range in
+// `splits` parts (partials to P + z ystride, ld m, when splits > 1).
+static int project_f64_fused(dh_handle_t h, int m, int n, int d, int l, const double* Xi,
+                             int ldxi, const double* Xj, int ldxj, const double* ni,
+                             const double* nj, const double* yi, const double* yj, double gamma,
+                             const double* O, int ldo, double beta, double* Y, int ldy, int ncb,
+                             int groups, int jps, int splits, long ystride) {
+    dim3 grid(dh_cdiv(m, 64), groups, splits);
+#define DH_DK(K)                                                                              \
+    dispatch_ncb<K>(h, ncb, grid, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, O, ldo, \
+                    beta, Y, ldy, jps, ystride)
+    if (d <= 32) return DH_DK(8);
+    if (d <= 64) return DH_DK(16);
+    if (d <= 128) return DH_DK(32);
+    return DH_DK(64);
+#undef DH_DK
+}
+
+// Column blocking of the fused FP64 pass: ncb blocks per group.
+static void f64_columns(int d, int l, int* ncb, int* groups) {
+    const int ncb_total = dh_cdiv(l, 16);
+    // Above 128 features the 17-block tile spills its accumulators: cap at 12.
+    int g = dh_cdiv(ncb_total, d > 128 ? 12 : 17);
+    *ncb = ncb_instance(dh_cdiv(ncb_total, g));
+    *groups = dh_cdiv(ncb_total, *ncb);
+}
+
+// Whole j range in one launch (no workspace for d <= 256).
 int dh_project_f64_impl(dh_handle_t h, int m, int n, int d, int l, const double* Xi, int ldxi,
                         const double* Xj, int ldxj, const double* ni, const double* nj,
                         const double* yi, const double* yj, double gamma, const double* O,
                         int ldo, double beta, double* Y, int ldy) {
     if (m == 0 || l == 0) return 0;
-    int ncb_total = dh_cdiv(l, 16);
-    // Above 128 features the 17-block tile spills its accumulators: cap at 12.
-    int groups = dh_cdiv(ncb_total, d > 128 ? 12 : 17);
-    int ncb = ncb_instance(dh_cdiv(ncb_total, groups));
-    groups = dh_cdiv(ncb_total, ncb);
-    dim3 grid(dh_cdiv(m, 64), groups);
-    if (d <= 32)
-        return dispatch_ncb<8>(h, ncb, grid, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma,
-                               O, ldo, beta, Y, ldy);
-    if (d <= 64)
-        return dispatch_ncb<16>(h, ncb, grid, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj,
-                                gamma, O, ldo, beta, Y, ldy);
-    if (d <= 128)
-        return dispatch_ncb<32>(h, ncb, grid, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj,
-                                gamma, O, ldo, beta, Y, ldy);
-    if (d <= 256)
-        return dispatch_ncb<64>(h, ncb, grid, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj,
-                                gamma, O, ldo, beta, Y, ldy);
-    return project_f64_blocked(h, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, O, ldo,
-                               beta, Y, ldy);
+    if (n == 0)  // K has no columns: Y = beta Y
+        return dh_gemm_impl<double>(h, false, false, m, l, 0, 1.0, O, 1, O, ldo, beta, Y, ldy, 0);
+    if (d > 256)
+        return project_f64_blocked(h, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, O,
+                                   ldo, beta, Y, ldy);
+    int ncb, groups;
+    f64_columns(d, l, &ncb, &groups);
+    return project_f64_fused(h, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, O, ldo,
+                             beta, Y, ldy, ncb, groups, n, 1, 0);
+}
+
+// Y = beta Y + sum over the splits z (in order) of P[z ystride + i + c m].
+__global__ __launch_bounds__(256) void sum_splits_f64_kernel(int m, int l, int splits,
+                                                             const double* __restrict__ P,
+                                                             long ystride, double beta,
+                                                             double* __restrict__ Y, int ldy) {
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long)m * l) return;
+    const int i = (int)(e % m), c = (int)(e / m);
+    double v = 0.0;
+    for (int z = 0; z < splits; ++z) v += P[z * ystride + e];
+    double* yp = Y + i + (size_t)c * ldy;
+    *yp = (beta == 0.0) ? v : v + beta * (*yp);
+}
+
+// As dh_project_f64_impl; a grid of at most half a workgroup per CU splits
+// the j range over more workgroups (partials in a workspace, summed in order).
+// The range finder's passes use it; dh_rbf_project_f64 keeps the one-launch
+// form, so its allocation-free twin returns the same bits.
+int dh_project_f64_split(dh_handle_t h, int m, int n, int d, int l, const double* Xi, int ldxi,
+                         const double* Xj, int ldxj, const double* ni, const double* nj,
+                         const double* yi, const double* yj, double gamma, const double* O,
+                         int ldo, double beta, double* Y, int ldy) {
+    if (m == 0 || l == 0 || n == 0 || d > 256)
+        return dh_project_f64_impl(h, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, O,
+                                   ldo, beta, Y, ldy);
+    int ncb, groups;
+    f64_columns(d, l, &ncb, &groups);
+    const int nch = dh_cdiv(n, 32);
+    const int splits = dh_j_splits(h->num_cu, (long)dh_cdiv(m, 64) * groups, nch);
+    if (splits == 1)
+        return project_f64_fused(h, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, O,
+                                 ldo, beta, Y, ldy, ncb, groups, n, 1, 0);
+    const int jps = dh_cdiv(nch, splits) * 32;
+    const long ystride = (long)m * l;
+    double* P = (double*)dh_ws(h, DH_WS_SPLITN, sizeof(double) * (size_t)ystride * splits);
+    if (!P) return DH_ERR_ALLOC;
+    DH_CHECK_STATUS(project_f64_fused(h, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, O,
+                                      ldo, 0.0, P, m, ncb, groups, jps, splits, ystride));
+    sum_splits_f64_kernel<<<dh_cdiv(ystride, 256), 256, 0, h->stream>>>(m, l, splits, P, ystride,
+                                                                        beta, Y, ldy);
+    DH_CHECK_LAUNCH();
+    return 0;
 }
 
 extern "C" int dh_rbf_project_f64(dh_handle_t h, int m, int n, int d, int l, const double* Xi,
