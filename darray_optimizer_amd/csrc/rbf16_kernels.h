@@ -98,8 +98,9 @@ __device__ __forceinline__ i32x8 join16(bf16x8 lo, bf16x8 hi) {
 // Accumulators stay in registers for the whole j loop; Y is written once.
 //
 // The exponent is c2 s + (a_i + b_j) with s = x_i . x_j, c2 = 2 gamma log2(e)
-// and a = -gamma log2(e) |x|^2; the exp's clamp to [0, 1] stands in for
-// clamping d2 at 0.
+// and a = -gamma log2(e) |x|^2, evaluated as c2 (a_i / c2 + s) + b_j with
+// a_i / c2 the distance MFMA's starting accumulator (16-bit distances); the
+// exp's clamp to [0, 1] stands in for clamping d2 at 0.
 //
 // D8: the distance runs in FP8 e4m3 instead (points stored as one byte per
 // feature, each row scaled by a power of two 1/sinv_i so that its largest
@@ -149,7 +150,7 @@ __device__ __forceinline__ void dma4(const void* g, void* lds) {
 // Wait for this wave's outstanding global (and global -> LDS) loads only.
 __device__ __forceinline__ void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
-template <int DK, int NCB, int WAVES, bool F16, int RB, bool D8 = false>
+template <int DK, int NCB, int WAVES, bool F16, int RB, bool D8 = false, bool PAIR = false>
 __device__ __forceinline__ void project_rows(
     int rblk, int cg, int m, int nchunks, int l, int ncb_total, const bf16x8* __restrict__ Xi,
     const bf16x8* __restrict__ Xj, const float* __restrict__ ai, const float* __restrict__ bj,
@@ -159,15 +160,25 @@ __device__ __forceinline__ void project_rows(
     using XS = XSlots<DK>;
     constexpr int XG = XS::XG, XROW = XS::XROW;
     constexpr int NBUF = 3;
+    // PAIR: waves w and w + WAVES/2 share their rows and take two adjacent
+    // column groups; each computes the distance and K of one 16-point half
+    // of the chunk (jb = its half) and the two exchange their halves of the
+    // projection's A operand through sK, so no distance is computed twice.
+    constexpr int NO = PAIR ? 2 * NCB : NCB;  // Omega blocks staged per chunk
+    constexpr int NJB = PAIR ? 1 : 2;         // distance blocks per wave and chunk
+    static_assert(!PAIR || (RB == 2 && WAVES % 2 == 0), "pairs: RB = 2, even wave count");
     __shared__ bf16x8 sX[NBUF][XS::SLOTS];
-    __shared__ bf16x8 sO[NBUF][NCB * 64];
+    __shared__ bf16x8 sO[NBUF][NO * 64];
     __shared__ float sB[NBUF][64];
     __shared__ float sS[D8 ? NBUF : 1][64];
+    __shared__ bf16x8 sK[PAIR ? 2 : 1][PAIR ? WAVES / 2 : 1][RB][64];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, li = lane & 15;
-    const int cb0 = cg * NCB;
-    const int i0 = (rblk * WAVES + wave) * 16 * RB;
+    const int half = PAIR ? wave / (WAVES / 2) : 0;         // wave-uniform
+    const int wrow = PAIR ? wave % (WAVES / 2) : wave;      // row slot of the wave
+    const int cb0 = PAIR ? (2 * cg + half) * NCB : cg * NCB;
+    const int i0 = (rblk * (PAIR ? WAVES / 2 : WAVES) + wrow) * 16 * RB;
 
     // Group of a row that lane (li, g) holds at operand slot q: 16-bit
     // k-step q covers groups 4q .. 4q+3 (one per g); FP8 k-step q/2 covers
@@ -185,6 +196,16 @@ __device__ __forceinline__ void project_rows(
         }
         arow[ib] = (row < m) ? ai[row] : 0.f;
         crow[ib] = D8 ? c2 * ((row < m) ? si[row] : 0.f) : c2;
+    }
+    // 16-bit distances start their accumulation at a_i / c2 (the first
+    // MFMA's accumulator input), so the exponent is c2 s + b_j: one VALU
+    // operation less per element.  FP8 distances scale s per point pair and
+    // keep the sum a_i + b_j.
+    f32x4 ainit[RB];
+#pragma unroll
+    for (int ib = 0; ib < RB; ++ib) {
+        const float v = (D8 || c2 == 0.f) ? 0.f : arow[ib] / c2;
+        ainit[ib] = f32x4{v, v, v, v};
     }
     f32x4 acc[RB][NCB];
 #pragma unroll
@@ -207,44 +228,48 @@ __device__ __forceinline__ void project_rows(
                 dma16(xc + row * XG + col, &sX[buf][kk * 64]);
             }
         }
-        const bf16x8* oc = Op + ((size_t)t * ncb_total + cb0) * 64;
+        const bf16x8* oc = Op + ((size_t)t * ncb_total + (PAIR ? 2 * cg * NCB : cb0)) * 64;
 #pragma unroll
-        for (int k = 0; k < NCB; k += WAVES) {
+        for (int k = 0; k < NO; k += WAVES) {
             const int kk = k + wave;
-            if (NCB % WAVES == 0 || kk < NCB) dma16(oc + kk * 64 + lane, &sO[buf][kk * 64]);
+            if (NO % WAVES == 0 || kk < NO) dma16(oc + kk * 64 + lane, &sO[buf][kk * 64]);
         }
         if (wave == WAVES - 1) dma4(bj + (size_t)t * 32 + (lane & 31), &sB[buf][0]);
         if (D8 && wave == WAVES - 2) dma4(sj + (size_t)t * 32 + (lane & 31), &sS[D8 ? buf : 0][0]);
     };
     // Distance operands: all 2 x DK A-operand reads of a chunk are issued
     // before its MFMAs.
-    auto distance = [&](int buf, f32x4 (&s)[2][RB]) {
-        bf16x8 a[DK][2];
+    // (Pairs: only the wave's half, jb = half, as s[0].)
+    auto distance = [&](int buf, f32x4 (&s)[NJB][RB]) {
+        bf16x8 a[DK][NJB];
 #pragma unroll
         for (int q = 0; q < DK; ++q)
 #pragma unroll
-            for (int jb = 0; jb < 2; ++jb) a[q][jb] = sX[buf][(16 * jb + li) * XROW + grp(q)];
+            for (int jj = 0; jj < NJB; ++jj) {
+                const int jb = PAIR ? half : jj;
+                a[q][jj] = sX[buf][(16 * jb + li) * XROW + grp(q)];
+            }
 #pragma unroll
-        for (int jb = 0; jb < 2; ++jb)
+        for (int jj = 0; jj < NJB; ++jj)
 #pragma unroll
-            for (int ib = 0; ib < RB; ++ib) s[jb][ib] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int ib = 0; ib < RB; ++ib) s[jj][ib] = ainit[ib];
         if constexpr (D8) {
 #pragma unroll
             for (int q = 0; q < DK; q += 2)
 #pragma unroll
-                for (int jb = 0; jb < 2; ++jb)
+                for (int jj = 0; jj < NJB; ++jj)
 #pragma unroll
                     for (int ib = 0; ib < RB; ++ib)
-                        s[jb][ib] = mfma_fp8(join16(a[q][jb], a[q + 1][jb]),
-                                             join16(xb[ib][q], xb[ib][q + 1]), s[jb][ib]);
+                        s[jj][ib] = mfma_fp8(join16(a[q][jj], a[q + 1][jj]),
+                                             join16(xb[ib][q], xb[ib][q + 1]), s[jj][ib]);
         } else {
 #pragma unroll
             for (int ks = 0; ks < DK; ++ks)
 #pragma unroll
-                for (int jb = 0; jb < 2; ++jb)
+                for (int jj = 0; jj < NJB; ++jj)
 #pragma unroll
                     for (int ib = 0; ib < RB; ++ib)
-                        s[jb][ib] = mfma16<F16>(a[ks][jb], xb[ib][ks], s[jb][ib]);
+                        s[jj][ib] = mfma16<F16>(a[ks][jj], xb[ib][ks], s[jj][ib]);
         }
     };
     // K = clamp(exp2(c2 s + a_i + b_j), 0, 1) as the projection's A operand.
@@ -252,19 +277,23 @@ __device__ __forceinline__ void project_rows(
     // the compiler turns into v_cvt_pk_f16_f32 (with the [0, 1] clamp folded
     // into v_exp_f32); element-wise FP16 conversions cost two more
     // instructions per pair.  BF16 pairs are packed either way.
-    auto epilogue = [&](int buf, const f32x4 (&s)[2][RB], bf16x8 (&kb)[RB]) {
+    // (Pairs: the wave's half only, elements 0..3 of kb; the other half is
+    // the partner's, see publish / gather.)
+    auto epilogue = [&](int buf, const f32x4 (&s)[NJB][RB], bf16x8 (&kb)[RB]) {
         float ev[RB][8];
 #pragma unroll
-        for (int jb = 0; jb < 2; ++jb)
+        for (int jj = 0; jj < NJB; ++jj)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
+                const int jb = PAIR ? half : jj;
                 float bv = sB[buf][16 * jb + 4 * g + r];
                 float sj = D8 ? sS[D8 ? buf : 0][16 * jb + 4 * g + r] : 1.f;
 #pragma unroll
                 for (int ib = 0; ib < RB; ++ib) {
                     float cc = D8 ? crow[ib] * sj : crow[ib];
-                    float e = __builtin_amdgcn_exp2f(fmaf(cc, s[jb][ib][r], arow[ib] + bv));
-                    ev[ib][4 * jb + r] = __builtin_amdgcn_fmed3f(e, 0.f, 1.f);
+                    float e = __builtin_amdgcn_exp2f(
+                        fmaf(cc, s[jj][ib][r], D8 ? arow[ib] + bv : bv));
+                    ev[ib][4 * jj + r] = __builtin_amdgcn_fmed3f(e, 0.f, 1.f);
                 }
             }
 #pragma unroll
@@ -273,32 +302,53 @@ __device__ __forceinline__ void project_rows(
                 typedef float f32x2 __attribute__((ext_vector_type(2)));
                 typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
                 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-                u32x4 w;
+                u32x4 w = {0u, 0u, 0u, 0u};
 #pragma unroll
-                for (int e = 0; e < 4; ++e)
+                for (int e = 0; e < 2 * NJB; ++e)
                     w[e] = __builtin_bit_cast(
                         unsigned,
                         __builtin_convertvector((f32x2{ev[ib][2 * e], ev[ib][2 * e + 1]}), f16x2));
                 kb[ib] = __builtin_bit_cast(bf16x8, w);
             } else {
 #pragma unroll
-                for (int e = 0; e < 8; ++e) kb[ib][e] = to16<false>(ev[ib][e]);
+                for (int e = 0; e < 4 * NJB; ++e) kb[ib][e] = to16<false>(ev[ib][e]);
             }
+        }
+    };
+    // Pairs: the wave's half of K (elements 0..3 of kb) goes to elements
+    // 4 half .. 4 half + 3 of the pair's slot in sK[par]; after the next
+    // barrier both waves read the whole A operand back.
+    auto publish = [&](int par, const bf16x8 (&kb)[RB]) {
+        if constexpr (PAIR) {
+            typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+            typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+            for (int ib = 0; ib < RB; ++ib) {
+                const u32x4 w = __builtin_bit_cast(u32x4, kb[ib]);
+                *(u32x2*)((unsigned*)&sK[par][wrow][ib][lane] + 2 * half) = u32x2{w[0], w[1]};
+            }
+        }
+    };
+    auto gather = [&](int par, bf16x8 (&kb)[RB]) {
+        if constexpr (PAIR) {
+#pragma unroll
+            for (int ib = 0; ib < RB; ++ib) kb[ib] = sK[par][wrow][ib][lane];
         }
     };
     // Projection B operands go through a ring of PF registers: the read of
     // column block cb + PF is issued before the MFMAs of block cb (an eight-deep
     // ring measured the same at d = 128, l = 272).
     constexpr int PF = NCB < 4 ? NCB : 4;
+    const int ob = half * NCB * 64;  // the wave's Omega blocks in sO
     auto project_start = [&](int buf, bf16x8 (&ring)[PF]) {
 #pragma unroll
-        for (int p = 0; p < PF; ++p) ring[p] = sO[buf][p * 64 + lane];
+        for (int p = 0; p < PF; ++p) ring[p] = sO[buf][ob + p * 64 + lane];
     };
     auto project = [&](int buf, const bf16x8 (&kb)[RB], bf16x8 (&ring)[PF]) {
 #pragma unroll
         for (int cb = 0; cb < NCB; ++cb) {
             const bf16x8 bo = ring[cb % PF];
-            if (cb + PF < NCB) ring[cb % PF] = sO[buf][(cb + PF) * 64 + lane];
+            if (cb + PF < NCB) ring[cb % PF] = sO[buf][ob + (cb + PF) * 64 + lane];
 #pragma unroll
             for (int ib = 0; ib < RB; ++ib) acc[ib][cb] = mfma16<F16>(kb[ib], bo, acc[ib][cb]);
         }
@@ -310,16 +360,20 @@ __device__ __forceinline__ void project_rows(
     // epilogue), chunk t-1 in (t-1)%3 (read by the projection), chunk t+1
     // loading into (t+1)%3, whose last reader (the projection of chunk t-2)
     // finished before the barrier that closed step t-1.
+    // Pairs: the halves of chunk t are published to sK[t & 1] in step t and
+    // gathered in step t + 1 (after the barrier between); sK[t & 1] is
+    // rewritten in step t + 2, after both waves read it in step t + 1.
     const int last = nchunks - 1;
     stage(0, 0);
     wait_vm();
     __syncthreads();
-    f32x4 s[2][RB];
+    f32x4 s[NJB][RB];
     bf16x8 kb_prev[RB], kb[RB], ring[PF];
     // Step 0 (no projection yet), then steps 1 .. nchunks-1 as one basic block.
     stage(min(1, last), 1);
     distance(0, s);
     epilogue(0, s, kb_prev);
+    publish(0, kb_prev);
     wait_vm();
     __syncthreads();
     int buf = 1;  // buffer of chunk t
@@ -328,8 +382,10 @@ __device__ __forceinline__ void project_rows(
         const int pbuf = (buf == 0) ? NBUF - 1 : buf - 1;   // chunk t-1
         stage(min(t + 1, last), nbuf);
         project_start(pbuf, ring);
+        gather((t - 1) & 1, kb_prev);
         distance(buf, s);
         epilogue(buf, s, kb);
+        publish(t & 1, kb);
         project(pbuf, kb_prev, ring);
 #pragma unroll
         for (int ib = 0; ib < RB; ++ib) kb_prev[ib] = kb[ib];
@@ -337,6 +393,7 @@ __device__ __forceinline__ void project_rows(
         __syncthreads();
         buf = nbuf;
     }
+    gather(last & 1, kb_prev);
     project_start(buf == 0 ? NBUF - 1 : buf - 1, ring);
     project(buf == 0 ? NBUF - 1 : buf - 1, kb_prev, ring);  // last chunk, after the final barrier
 
@@ -359,7 +416,7 @@ __device__ __forceinline__ void project_rows(
 // Grid: x = row block, y = column group, z = split of the j range: split z
 // takes chunks [z cps, min((z + 1) cps, nchunks)) and writes its partial
 // product to Y + z * ystride (one split: the whole range into Y).
-template <int DK, int NCB, int WAVES, bool F16, int RB, bool D8 = false>
+template <int DK, int NCB, int WAVES, bool F16, int RB, bool D8 = false, bool PAIR = false>
 __global__ __launch_bounds__(WAVES * 64, 1) void rbf_project_bf16_kernel(
     int m, int nchunks, int l, int ncb_total, const bf16x8* __restrict__ Xi,
     const bf16x8* __restrict__ Xj, const float* __restrict__ ai, const float* __restrict__ bj,
@@ -368,7 +425,7 @@ __global__ __launch_bounds__(WAVES * 64, 1) void rbf_project_bf16_kernel(
     int ldy, int cps, long ystride) {
     const int t0 = blockIdx.z * cps;
     const size_t p0 = (size_t)t0 * 32;
-    project_rows<DK, NCB, WAVES, F16, RB, D8>(
+    project_rows<DK, NCB, WAVES, F16, RB, D8, PAIR>(
         blockIdx.x, blockIdx.y, m, min(cps, nchunks - t0), l, ncb_total, Xi,
         Xj + p0 * XSlots<DK>::XG, ai, bj + p0, si, sj ? sj + p0 : nullptr, c2,
         Op + (size_t)t0 * ncb_total * 64, yi, Y + blockIdx.z * ystride, ldy);
@@ -406,6 +463,21 @@ int launch_bf16(dh_handle_t h, int m, int groups, int nchunks, int l, int ncb_to
                 long ystride) {
     const int cps = dh_cdiv(nchunks, splits);
     const int nz = dh_cdiv(nchunks, cps);
+    // An even number of 17-block column groups (l > 272) runs as wave pairs
+    // (rows shared, each distance computed once); up to 128 features the
+    // pair's LDS ring fits.
+    if constexpr (NCB == 17 && DK <= 4) {
+        if (groups % 2 == 0) {
+            dh_timing_begin(h);
+            rbf_project_bf16_kernel<DK, NCB, 8, F16, 2, false, true>
+                <<<dim3(dh_cdiv(m, 4 * 32), groups / 2, nz), 512, 0, h->stream>>>(
+                    m, nchunks, l, ncb_total, Xi, Xj, ai, bj, nullptr, nullptr, c2, Op, yi, Y,
+                    ldy, cps, ystride);
+            dh_timing_end(h);
+            DH_CHECK_LAUNCH();
+            return 0;
+        }
+    }
     auto run = [&](int sched) {
 #define DH_RUN(W, RB)                                                                        \
     rbf_project_bf16_kernel<DK, NCB, W, F16, RB>                                             \

@@ -56,7 +56,10 @@ def dev_vec(v):
 
 
 SHAPES = [(1, 1, 1, 1), (64, 64, 64, 16), (100, 77, 13, 9), (300, 257, 130, 138),
-          (513, 1000, 128, 272), (200, 190, 200, 300), (96, 500, 256, 40)]
+          (513, 1000, 128, 272), (200, 190, 200, 300), (96, 500, 256, 40),
+          # l > 272 with d <= 128: wave pairs over two column groups (and, for n = 2100,
+          # a j range split in four)
+          (130, 257, 64, 300), (300, 2100, 100, 544)]
 
 
 @pytest.mark.parametrize("m,n,d,l", SHAPES)
@@ -213,7 +216,8 @@ def test_e4m3_rounding_known_values():
 
 
 @pytest.mark.parametrize("m,n,d,l", [(64, 64, 13, 16), (300, 257, 64, 138), (200, 333, 128, 40),
-                                     (513, 1000, 130, 272), (96, 500, 256, 300)])
+                                     (513, 1000, 130, 272), (96, 500, 256, 300),
+                                     (200, 700, 64, 544)])
 def test_rbf_project_fp8(dev, m, n, d, l):
     import torch
     rng = np.random.default_rng(11 * m + n + d + l)
