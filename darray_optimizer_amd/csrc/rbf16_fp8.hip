@@ -4,15 +4,16 @@
 
 namespace {
 
-// An even number of 17-block column groups (l > 272) runs as pairs: waves w
-// and w + 4 share 32 rows and take two adjacent groups, each computing one
-// half of every chunk's distances and K (no group recomputes another's).
+// An even number of 17-block column groups (l > 272) at 256 features runs as
+// pairs: waves w and w + 4 share 32 rows and take two adjacent groups, each
+// computing one half of every chunk's distances and K (no group recomputes
+// another's).
 template <int DK, int NCB>
 void launch_fp8(dh_handle_t h, int m, int groups, int nchunks, int l, int ncb_total,
                 const bf16x8* Xi, const bf16x8* Xj, const float* ai, const float* bj,
                 const float* si, const float* sj, float c2, const bf16x8* Op, const double* yi, double* Y,
                 int ldy) {
-    if constexpr (NCB == 17) {
+    if constexpr (NCB == 17 && DK == 4) {  // 256 features (pairs measured slower at 128)
         if (groups % 2 == 0) {
             rbf_project_bf16_kernel<DK, NCB, 8, false, 2, true, true>
                 <<<dim3(dh_cdiv(m, 4 * 32), groups / 2), 512, 0, h->stream>>>(

@@ -124,26 +124,34 @@ struct XSlots {
     static constexpr int SLOTS = INSTR * 64;
 };
 
-// 16-byte (or 4-byte) direct global -> LDS copy by one wave: lane L's value
-// lands at lds + L * size; lds must be the same for all lanes of the wave.
+// 16-byte (or 4-byte) direct global -> LDS copy by one wave from the uniform
+// address gbase (scalar registers) plus lane L's byte offset voff; lane L's
+// value lands at lds + L * size, lds the same for all lanes of the wave.
 // Written as inline assembly (M0 = LDS base, one wait state, then the load)
 // so that the compiler does not make every later LDS read wait for the copy:
 // the copies of a step go to a buffer nobody reads in that step, and
-// wait_vm() + the barrier at the end of the step publish them.
+// wait_vm() + the barrier at the end of the step publish them.  The scalar
+// base leaves no per-lane address arithmetic to the vector ALU.
 #pragma clang diagnostic push
 #pragma clang diagnostic ignored "-Winline-asm"
-__device__ __forceinline__ void dma16(const void* g, void* lds) {
+__device__ __forceinline__ const void* uniform_ptr(const void* p) {
+    const uintptr_t v = (uintptr_t)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return (const void*)(((uintptr_t)hi << 32) | lo);
+}
+__device__ __forceinline__ void dma16(const void* gbase, unsigned voff, void* lds) {
     unsigned base = (unsigned)(uintptr_t)(__attribute__((address_space(3))) void*)lds;
-    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, %2"
                  :
-                 : "v"(g), "s"(__builtin_amdgcn_readfirstlane(base))
+                 : "v"(voff), "s"(__builtin_amdgcn_readfirstlane(base)), "s"(uniform_ptr(gbase))
                  : "memory", "m0");
 }
-__device__ __forceinline__ void dma4(const void* g, void* lds) {
+__device__ __forceinline__ void dma4(const void* gbase, unsigned voff, void* lds) {
     unsigned base = (unsigned)(uintptr_t)(__attribute__((address_space(3))) void*)lds;
-    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off"
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, %2"
                  :
-                 : "v"(g), "s"(__builtin_amdgcn_readfirstlane(base))
+                 : "v"(voff), "s"(__builtin_amdgcn_readfirstlane(base)), "s"(uniform_ptr(gbase))
                  : "memory", "m0");
 }
 #pragma clang diagnostic pop
@@ -173,7 +181,8 @@ __device__ __forceinline__ void project_rows(
     __shared__ float sS[D8 ? NBUF : 1][64];
     __shared__ bf16x8 sK[PAIR ? 2 : 1][PAIR ? WAVES / 2 : 1][RB][64];
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar addressing
     const int g = lane >> 4, li = lane & 15;
     const int half = PAIR ? wave / (WAVES / 2) : 0;         // wave-uniform
     const int wrow = PAIR ? wave % (WAVES / 2) : wave;      // row slot of the wave
@@ -213,29 +222,37 @@ __device__ __forceinline__ void project_rows(
 #pragma unroll
         for (int cb = 0; cb < NCB; ++cb) acc[ib][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    // Issue the loads of chunk t into buffer buf (instructions spread over the waves).
+    // Issue the loads of chunk t into buffer buf (instructions spread over the
+    // waves).  Byte offsets of the wave's lanes in the padded image of Xj's
+    // chunk, fixed for the whole loop (padding slots load row 0, never read).
+    constexpr int NXK = (XS::INSTR + WAVES - 1) / WAVES;
+    unsigned offx[NXK];
+#pragma unroll
+    for (int k = 0; k < XS::INSTR; k += WAVES) {
+        int slot = (k + wave) * 64 + lane, row = slot / XROW, col = slot % XROW;
+        if (row >= 32 || col >= XG) {
+            row = 0;
+            col = 0;
+        }
+        offx[k / WAVES] = (unsigned)(row * XG + col) * 16u;
+    }
     auto stage = [&](int t, int buf) {
         const bf16x8* xc = Xj + (size_t)t * 32 * XG;
 #pragma unroll
         for (int k = 0; k < XS::INSTR; k += WAVES) {
             const int kk = k + wave;
-            if (XS::INSTR % WAVES == 0 || kk < XS::INSTR) {
-                int slot = kk * 64 + lane, row = slot / XROW, col = slot % XROW;
-                if (row >= 32 || col >= XG) {  // padding / rounding slots: never read
-                    row = 0;
-                    col = 0;
-                }
-                dma16(xc + row * XG + col, &sX[buf][kk * 64]);
-            }
+            if (XS::INSTR % WAVES == 0 || kk < XS::INSTR)
+                dma16(xc, offx[k / WAVES], &sX[buf][kk * 64]);
         }
         const bf16x8* oc = Op + ((size_t)t * ncb_total + (PAIR ? 2 * cg * NCB : cb0)) * 64;
 #pragma unroll
         for (int k = 0; k < NO; k += WAVES) {
             const int kk = k + wave;
-            if (NO % WAVES == 0 || kk < NO) dma16(oc + kk * 64 + lane, &sO[buf][kk * 64]);
+            if (NO % WAVES == 0 || kk < NO) dma16(oc + kk * 64, (unsigned)lane * 16u, &sO[buf][kk * 64]);
         }
-        if (wave == WAVES - 1) dma4(bj + (size_t)t * 32 + (lane & 31), &sB[buf][0]);
-        if (D8 && wave == WAVES - 2) dma4(sj + (size_t)t * 32 + (lane & 31), &sS[D8 ? buf : 0][0]);
+        if (wave == WAVES - 1) dma4(bj + (size_t)t * 32, (unsigned)(lane & 31) * 4u, &sB[buf][0]);
+        if (D8 && wave == WAVES - 2)
+            dma4(sj + (size_t)t * 32, (unsigned)(lane & 31) * 4u, &sS[D8 ? buf : 0][0]);
     };
     // Distance operands: all 2 x DK A-operand reads of a chunk are issued
     // before its MFMAs.
