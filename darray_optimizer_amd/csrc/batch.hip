@@ -126,11 +126,17 @@ struct Plan {
     hipGraphExec_t exec = nullptr;
     int* infos = nullptr;   // 2 per problem: status of the latest Q / W CholeskyQR step
     int* sticky = nullptr;  // 1 per problem: some CholeskyQR step broke down
+    // Side streams (default priority) and events of the captured fork / join
+    // that runs the tile classes of a pass side by side.
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> events;
 
     ~Plan() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (mem) (void)hipFree(mem);
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        for (hipStream_t s : side) (void)hipStreamDestroy(s);
     }
 };
 
@@ -528,6 +534,51 @@ int build_plan(dh_handle_t h, Plan& P, int prec, int count, const dh_lra_problem
         }
         return 0;
     };
+    // The classes of a pass are independent (different problems): up to four
+    // run side by side, biggest first, on the handle's stream and up to three
+    // side streams forked from it and joined back inside the capture.
+    std::vector<int> order(classes.size());
+    for (size_t c = 0; c < classes.size(); ++c) order[c] = (int)c;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return (long)classes[a].count * classes[a].cls.dk * classes[a].cls.ncb >
+               (long)classes[b].count * classes[b].cls.dk * classes[b].cls.ncb;
+    });
+    const int lanes = std::min<int>(4, (int)classes.size());
+    for (int i = 1; i < lanes; ++i) {
+        hipStream_t s = nullptr;
+        DH_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        P.side.push_back(s);
+    }
+    for (int i = 0; i < lanes; ++i) {
+        hipEvent_t e = nullptr;
+        DH_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        P.events.push_back(e);
+    }
+    // Projections of one pass: fork, classes round robin over the lanes, join.
+    auto project_pass = [&](int pass, int t) -> int {
+        hipStream_t main = h->stream;
+        if (lanes > 1) {
+            if (hipEventRecord(P.events[0], main) != hipSuccess) return DH_ERR_HIP;
+            for (hipStream_t s : P.side)
+                if (hipStreamWaitEvent(s, P.events[0], 0) != hipSuccess) return DH_ERR_HIP;
+        }
+        int st = 0;
+        for (size_t k = 0; k < order.size() && !st; ++k) {
+            const int c = order[k];
+            h->stream = (k % lanes == 0) ? main : P.side[k % lanes - 1];
+            st = dh_bf16_project_persistent(h, classes[c].cls, d_proj + t * count,
+                                            d_items + classes[c].first, classes[c].count,
+                                            d_counters + pass * (int)classes.size() + c);
+        }
+        h->stream = main;
+        if (lanes > 1) {
+            for (int i = 1; i < lanes; ++i) {
+                if (hipEventRecord(P.events[i], P.side[i - 1]) != hipSuccess) return DH_ERR_HIP;
+                if (hipStreamWaitEvent(main, P.events[i], 0) != hipSuccess) return DH_ERR_HIP;
+            }
+        }
+        return st;
+    };
     DH_CHECK_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     int st = 0;
     do {
@@ -539,10 +590,7 @@ int build_plan(dh_handle_t h, Plan& P, int prec, int count, const dh_lra_problem
             const int t = pass == 0 ? 0 : 1;  // operand Omega / Q, output Q / W
             if ((st = dh_bf16_pack_batched(h, d_pack + t * count, count, max_pack_threads, f16)))
                 break;
-            for (size_t c = 0; c < classes.size() && !st; ++c)
-                st = dh_bf16_project_persistent(h, classes[c].cls, d_proj + t * count,
-                                                d_items + classes[c].first, classes[c].count,
-                                                d_counters + pass * (int)classes.size() + (int)c);
+            st = project_pass(pass, t);
             if (st || pass == passes - 1) break;
             if ((st = orth(t))) break;
             if (pass >= 1) {
