@@ -636,6 +636,12 @@ extern "C" int dh_lra_batched(dh_handle_t h, int prec, int count, const dh_lra_p
             return dh_fail(-4, "%s: problem %d: 16-bit batches need d <= 256 and l <= 272", name,
                            i);
     }
+    // The batch runs as its staged graph.  A single persistent launch of every
+    // stage was tried and hung on the device; it is not available, and asking
+    // for it is an error rather than a silent fallback.
+    if (const char* e = std::getenv("DH_BATCH_SCHEDULE"); e && std::strcmp(e, "single") == 0)
+        return dh_fail(DH_ERR_UNSUPPORTED, "%s: DH_BATCH_SCHEDULE=single is not available",
+                       name);
     std::vector<int> st(count, 0);
     int first = 0;
     if (prec != DH_PREC_BF16 && prec != DH_PREC_F16) {

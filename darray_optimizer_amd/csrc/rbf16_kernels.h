@@ -76,6 +76,87 @@ __device__ __forceinline__ i32x8 join16(bf16x8 lo, bf16x8 hi) {
     return i32x8{a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
 }
 
+// One wave per point row (rows >= n are written as zeros):
+// Xb = round16((X - mean) * scale), padded with zeros to dpad features, and
+// a[i] = -gamma log2(e) |Xb_i|^2 (gamma of the scaled points).
+template <bool F16>
+__device__ __forceinline__ void convert_row(int row, int lane, int n, int d, int dpad,
+                                            const double* __restrict__ X, int ldx,
+                                            const double* __restrict__ mean, double scale,
+                                            float neg_gl2e, __bf16* __restrict__ Xb,
+                                            float* __restrict__ a) {
+    float s = 0.f;
+    for (int f = lane; f < dpad; f += 64) {
+        __bf16 v = to16<F16>(0.0f);
+        if (row < n && f < d) v = to16<F16>((float)((X[f + (size_t)row * ldx] - mean[f]) * scale));
+        Xb[(size_t)row * dpad + f] = v;
+        float fv = from16<F16>(v);
+        s += fv * fv;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) a[row] = (row < n) ? neg_gl2e * s : 0.f;
+}
+
+// Packed operand of the projection MFMA.  For chunk t (32 points), column
+// block cb (16 columns) and lane L = 16 g + li, the 8 BF16 values are
+//   Op[t][cb][L][e] = y[j] * O[j][16 cb + li],  j = 32 t + 16 (e / 4) + 4 g + e % 4,
+// zero outside n x l.  One thread per (chunk, column): it reads 32 contiguous
+// doubles of the column and writes four 16-byte groups.
+template <bool F16>
+__device__ __forceinline__ void pack_one(long idx, int n, int l, int nchunks, int ncb_total,
+                                         const double* __restrict__ O, int ldo,
+                                         const double* __restrict__ y, bf16x8* __restrict__ Op) {
+    long ncols = (long)ncb_total * 16;
+    if (idx >= (long)nchunks * ncols) return;
+    int t = (int)(idx / ncols), c = (int)(idx % ncols);
+    int cb = c >> 4, li = c & 15;
+    float v[32];
+    const int j0 = 32 * t;
+    const double* src = O + j0 + (size_t)min(c, l - 1) * ldo;
+    const bool vec = c < l && j0 + 32 <= n && ((reinterpret_cast<uintptr_t>(src) & 15) == 0) &&
+                     (!y || (reinterpret_cast<uintptr_t>(y + j0) & 15) == 0);
+    if (vec) {
+        // Whole chunk in range and 16-byte aligned: 16 unconditional 2-double
+        // loads (loads under per-element branches would be issued one by one).
+        double o[32];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const double2 w = reinterpret_cast<const double2*>(src)[q];
+            o[2 * q] = w.x;
+            o[2 * q + 1] = w.y;
+        }
+        if (y) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const double2 w = reinterpret_cast<const double2*>(y + j0)[q];
+                o[2 * q] *= w.x;
+                o[2 * q + 1] *= w.y;
+            }
+        }
+#pragma unroll
+        for (int jj = 0; jj < 32; ++jj) v[jj] = (float)o[jj];
+    } else {
+#pragma unroll
+        for (int jj = 0; jj < 32; ++jj) {
+            int j = j0 + jj;
+            double o = 0.0;
+            if (j < n && c < l) {
+                o = O[j + (size_t)c * ldo];
+                if (y) o *= y[j];
+            }
+            v[jj] = (float)o;
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        bf16x8 w;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) w[e] = to16<F16>(v[16 * (e >> 2) + 4 * g + (e & 3)]);
+        Op[((size_t)t * ncb_total + cb) * 64 + 16 * g + li] = w;
+    }
+}
+
 // ------------------------------------------------------------ main kernel
 //
 // Workgroup = WAVES waves; wave w owns 16*RB rows i (RB 16-row MFMA blocks).
@@ -158,13 +239,28 @@ __device__ __forceinline__ void dma4(const void* gbase, unsigned voff, void* lds
 // Wait for this wave's outstanding global (and global -> LDS) loads only.
 __device__ __forceinline__ void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+// LDS of one workgroup of project_rows: the 3-deep ring of Xj chunks (sX),
+// packed Omega blocks (sO), norms b_j (sB) and FP8 scales (sS), and the
+// exchange slots of wave pairs (sK).  Kernels declare it __shared__ or carve
+// it from a larger arena.
+template <int DK, int NCB, int WAVES, int RB, bool D8, bool PAIR>
+struct ProjLds {
+    static constexpr int NBUF = 3;
+    static constexpr int NO = PAIR ? 2 * NCB : NCB;  // Omega blocks staged per chunk
+    bf16x8 sX[NBUF][XSlots<DK>::SLOTS];
+    bf16x8 sO[NBUF][NO * 64];
+    float sB[NBUF][64];
+    float sS[D8 ? NBUF : 1][64];
+    bf16x8 sK[PAIR ? 2 : 1][PAIR ? WAVES / 2 : 1][PAIR ? RB : 1][PAIR ? 64 : 1];
+};
+
 template <int DK, int NCB, int WAVES, bool F16, int RB, bool D8 = false, bool PAIR = false>
 __device__ __forceinline__ void project_rows(
-    int rblk, int cg, int m, int nchunks, int l, int ncb_total, const bf16x8* __restrict__ Xi,
-    const bf16x8* __restrict__ Xj, const float* __restrict__ ai, const float* __restrict__ bj,
-    const float* __restrict__ si, const float* __restrict__ sj, float c2,
-    const bf16x8* __restrict__ Op, const double* __restrict__ yi, double* __restrict__ Y,
-    int ldy) {
+    ProjLds<DK, NCB, WAVES, RB, D8, PAIR>& lds, int rblk, int cg, int m, int nchunks, int l,
+    int ncb_total, const bf16x8* __restrict__ Xi, const bf16x8* __restrict__ Xj,
+    const float* __restrict__ ai, const float* __restrict__ bj, const float* __restrict__ si,
+    const float* __restrict__ sj, float c2, const bf16x8* __restrict__ Op,
+    const double* __restrict__ yi, double* __restrict__ Y, int ldy) {
     using XS = XSlots<DK>;
     constexpr int XG = XS::XG, XROW = XS::XROW;
     constexpr int NBUF = 3;
@@ -175,11 +271,11 @@ __device__ __forceinline__ void project_rows(
     constexpr int NO = PAIR ? 2 * NCB : NCB;  // Omega blocks staged per chunk
     constexpr int NJB = PAIR ? 1 : 2;         // distance blocks per wave and chunk
     static_assert(!PAIR || (RB == 2 && WAVES % 2 == 0), "pairs: RB = 2, even wave count");
-    __shared__ bf16x8 sX[NBUF][XS::SLOTS];
-    __shared__ bf16x8 sO[NBUF][NO * 64];
-    __shared__ float sB[NBUF][64];
-    __shared__ float sS[D8 ? NBUF : 1][64];
-    __shared__ bf16x8 sK[PAIR ? 2 : 1][PAIR ? WAVES / 2 : 1][RB][64];
+    auto& sX = lds.sX;
+    auto& sO = lds.sO;
+    auto& sB = lds.sB;
+    auto& sS = lds.sS;
+    auto& sK = lds.sK;
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar addressing
@@ -442,8 +538,9 @@ __global__ __launch_bounds__(WAVES * 64, 1) void rbf_project_bf16_kernel(
     int ldy, int cps, long ystride) {
     const int t0 = blockIdx.z * cps;
     const size_t p0 = (size_t)t0 * 32;
+    __shared__ ProjLds<DK, NCB, WAVES, RB, D8, PAIR> lds;
     project_rows<DK, NCB, WAVES, F16, RB, D8, PAIR>(
-        blockIdx.x, blockIdx.y, m, min(cps, nchunks - t0), l, ncb_total, Xi,
+        lds, blockIdx.x, blockIdx.y, m, min(cps, nchunks - t0), l, ncb_total, Xi,
         Xj + p0 * XSlots<DK>::XG, ai, bj + p0, si, sj ? sj + p0 : nullptr, c2,
         Op + (size_t)t0 * ncb_total * 64, yi, Y + blockIdx.z * ystride, ldy);
 }

@@ -13,6 +13,7 @@ __global__ __launch_bounds__(WAVES * 64, 1) void rbf_project_bf16_persistent(
     const DhProjJob* __restrict__ jobs, const DhProjItem* __restrict__ items, int nitems,
     int* __restrict__ counter) {
     __shared__ int s_item;
+    __shared__ ProjLds<DK, NCB, WAVES, RB, false, false> lds;
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
         __syncthreads();
@@ -21,7 +22,7 @@ __global__ __launch_bounds__(WAVES * 64, 1) void rbf_project_bf16_persistent(
         if (it >= nitems) break;
         const DhProjItem w = items[it];
         const DhProjJob& j = jobs[w.job];
-        project_rows<DK, NCB, WAVES, F16, RB>(w.rblk, w.cg, j.m, j.nchunks, j.l, j.ncb_pack,
+        project_rows<DK, NCB, WAVES, F16, RB>(lds, w.rblk, w.cg, j.m, j.nchunks, j.l, j.ncb_pack,
                                                 (const bf16x8*)j.xb, (const bf16x8*)j.xb, j.a, j.a,
                                                 nullptr, nullptr, j.c2, (const bf16x8*)j.Op, j.y, j.Y,
                                                 j.ldy);

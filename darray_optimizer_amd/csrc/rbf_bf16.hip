@@ -102,28 +102,6 @@ __global__ __launch_bounds__(1024) void colmean_final_kernel(int n, int d, int n
     }
 }
 
-// One wave per point row (rows >= n are written as zeros):
-// Xb = round16((X - mean) * scale), padded with zeros to dpad features, and
-// a[i] = -gamma log2(e) |Xb_i|^2 (gamma of the scaled points).
-template <bool F16>
-__device__ __forceinline__ void convert_row(int row, int lane, int n, int d, int dpad,
-                                            const double* __restrict__ X, int ldx,
-                                            const double* __restrict__ mean, double scale,
-                                            float neg_gl2e, __bf16* __restrict__ Xb,
-                                            float* __restrict__ a) {
-    float s = 0.f;
-    for (int f = lane; f < dpad; f += 64) {
-        __bf16 v = to16<F16>(0.0f);
-        if (row < n && f < d) v = to16<F16>((float)((X[f + (size_t)row * ldx] - mean[f]) * scale));
-        Xb[(size_t)row * dpad + f] = v;
-        float fv = from16<F16>(v);
-        s += fv * fv;
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (lane == 0) a[row] = (row < n) ? neg_gl2e * s : 0.f;
-}
-
 template <bool F16>
 __global__ __launch_bounds__(256) void convert_rows_kernel(int n, int nrows, int d, int dpad,
                                                            const double* __restrict__ X, int ldx,
@@ -222,64 +200,6 @@ __global__ __launch_bounds__(256) void convert_rows_fp8_kernel(int n, int nrows,
     }
 }
 
-// Packed operand of the projection MFMA.  For chunk t (32 points), column
-// block cb (16 columns) and lane L = 16 g + li, the 8 BF16 values are
-//   Op[t][cb][L][e] = y[j] * O[j][16 cb + li],  j = 32 t + 16 (e / 4) + 4 g + e % 4,
-// zero outside n x l.  One thread per (chunk, column): it reads 32 contiguous
-// doubles of the column and writes four 16-byte groups.
-template <bool F16>
-__device__ __forceinline__ void pack_one(long idx, int n, int l, int nchunks, int ncb_total,
-                                         const double* __restrict__ O, int ldo,
-                                         const double* __restrict__ y, bf16x8* __restrict__ Op) {
-    long ncols = (long)ncb_total * 16;
-    if (idx >= (long)nchunks * ncols) return;
-    int t = (int)(idx / ncols), c = (int)(idx % ncols);
-    int cb = c >> 4, li = c & 15;
-    float v[32];
-    const int j0 = 32 * t;
-    const double* src = O + j0 + (size_t)min(c, l - 1) * ldo;
-    const bool vec = c < l && j0 + 32 <= n && ((reinterpret_cast<uintptr_t>(src) & 15) == 0) &&
-                     (!y || (reinterpret_cast<uintptr_t>(y + j0) & 15) == 0);
-    if (vec) {
-        // Whole chunk in range and 16-byte aligned: 16 unconditional 2-double
-        // loads (loads under per-element branches would be issued one by one).
-        double o[32];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const double2 w = reinterpret_cast<const double2*>(src)[q];
-            o[2 * q] = w.x;
-            o[2 * q + 1] = w.y;
-        }
-        if (y) {
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const double2 w = reinterpret_cast<const double2*>(y + j0)[q];
-                o[2 * q] *= w.x;
-                o[2 * q + 1] *= w.y;
-            }
-        }
-#pragma unroll
-        for (int jj = 0; jj < 32; ++jj) v[jj] = (float)o[jj];
-    } else {
-#pragma unroll
-        for (int jj = 0; jj < 32; ++jj) {
-            int j = j0 + jj;
-            double o = 0.0;
-            if (j < n && c < l) {
-                o = O[j + (size_t)c * ldo];
-                if (y) o *= y[j];
-            }
-            v[jj] = (float)o;
-        }
-    }
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-        bf16x8 w;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) w[e] = to16<F16>(v[16 * (e >> 2) + 4 * g + (e & 3)]);
-        Op[((size_t)t * ncb_total + cb) * 64 + 16 * g + li] = w;
-    }
-}
 
 template <bool F16>
 __global__ __launch_bounds__(256) void pack_omega_kernel(int n, int l, int nchunks, int ncb_total,
