@@ -449,9 +449,10 @@ __device__ __forceinline__ void project_rows(
         }
     };
     // Projection B operands go through a ring of PF registers: the read of
-    // column block cb + PF is issued before the MFMAs of block cb (an eight-deep
-    // ring measured the same at d = 128, l = 272).
-    constexpr int PF = NCB < 4 ? NCB : 4;
+    // column block cb + PF is issued before the MFMAs of block cb.  Eight
+    // deep: 2.48 ms against 2.54 ms for four at d = 128, l = 272 (FP16,
+    // median of four rounds of 15 launches each).
+    constexpr int PF = NCB < 8 ? NCB : 8;
     const int ob = half * NCB * 64;  // the wave's Omega blocks in sO
     auto project_start = [&](int buf, bf16x8 (&ring)[PF]) {
 #pragma unroll
