@@ -31,6 +31,7 @@ enum dh_slot {
     DH_WS_SVM,         // vectors and partial sums of the barrier SVM solver
     DH_WS_KBLOCK,      // materialised K row block of the d > 256 FP64 pass
     DH_WS_SPLITN,      // partial products of a K-pass whose j range is split
+    DH_WS_PACK64,      // packed FP64 Omega / Q of the fused FP64 pass
     DH_WS_COUNT
 };
 

@@ -19,7 +19,7 @@
 int dh_project_f64_impl(dh_handle_t h, int m, int n, int d, int l, const double* Xi, int ldxi,
                         const double* Xj, int ldxj, const double* ni, const double* nj,
                         const double* yi, const double* yj, double gamma, const double* O,
-                        int ldo, double beta, double* Y, int ldy);
+                        int ldo, double beta, double* Y, int ldy, bool pack);
 
 static __device__ __align__(256) double scratch[DH_NOALLOC_MAX_POINTS];
 
@@ -83,7 +83,7 @@ extern "C" int dh_rbf_project_f64_noalloc(dh_handle_t h, int m, int n, int d, in
     void* timing = h->timing;
     h->timing = nullptr;
     const int st = dh_project_f64_impl(h, m, n, d, l, Xi, ldxi, Xj, ldxj, nrm, nrm + m, yi, yj,
-                                       gamma, O, ldo, beta, Y, ldy);
+                                       gamma, O, ldo, beta, Y, ldy, false);
     h->timing = timing;
     return st;
 }

@@ -7,6 +7,7 @@
 // (dkernel.h:33-42, dkernel.cu:194-231).  Here the distance GEMM and the
 // projection are two MFMA stages of one kernel and K stays in registers.
 #include "dh_common.h"
+#include "lds_dma.h"
 #include "mfma_tile.h"
 
 // ------------------------------------------------------------------ norms
@@ -143,8 +144,11 @@ extern "C" int dh_rbf_tile_f32(dh_handle_t h, int m, int n, int d, const float* 
 //      holds S^T[L/16+4r][L%16] = K[i=L%16][j=4r+L/16] (f64 accumulator
 //      layout), i.e. MFMA r contracts over j = 4r..4r+3; Omega is staged in
 //      LDS so that lane L reads the B operands of MFMAs 2h and 2h+1 of block
-//      cb as one 16-byte pair sO[cb][h][L] = (Omega[j0+4(2h)+L/16][c],
-//      Omega[j0+4(2h+1)+L/16][c]), c = c0 + 16cb + L%16.
+//      cb as one 16-byte pair sO[cb][h][L ^ 4(L/16)] = (Omega[j0+4(2h)+L/16][c],
+//      Omega[j0+4(2h+1)+L/16][c]), c = c0 + 16cb + L%16.  The XOR keeps each
+//      aligned group of 8 lanes on 8 consecutive pairs (conflict-free reads)
+//      and spreads the staging stores, whose 64 lanes cover 4 columns x 16
+//      points, over all banks (2 cycles per store instead of 8).
 // The chunks go through two LDS buffers: the global loads of chunk t+1 are
 // issued into registers before chunk t is computed and stored to the other
 // buffer after it, so one barrier per chunk orders both.  (With one buffer,
@@ -154,14 +158,69 @@ extern "C" int dh_rbf_tile_f32(dh_handle_t h, int m, int n, int d, const float* 
 // Labels are folded: yj scales Omega as it is staged, yi scales the output.
 // The projection accumulators (NCB blocks of 16 columns) stay in registers
 // for the whole j loop; Y is written once.
-template <int DK, int NCB>
+// exp(x) without branches for the kernel's exponents x = -gamma d2 <= 0
+// (valid up to x = 709): k = rint(x log2 e); r = (x - k LN2_HI) - k LN2_LO, a
+// Cody-Waite reduction with LN2_HI = ln 2 rounded to 32 significant bits (k
+// LN2_HI exact for |k| < 2^21) and LN2_LO = ln 2 - LN2_HI rounded to double;
+// e^r by its Taylor series to degree 13 in Horner form (truncation below
+// 2^-55 for |r| <= ln2 / 2); then 2^k by ldexp, which rounds subnormal
+// results and gives 0 below 2^-1075.  Within 1 ulp of libm's exp over
+// [-746, 0] (an exact-FMA emulation checked on 20000 points and the edges).
+__device__ __forceinline__ double exp_kernel_f64(double x) {
+    x = fmax(x, -746.0);
+    const double k = rint(x * 1.4426950408889634);
+    double r = fma(-k, 0x1.62e42ff000000p-1, x);
+    r = fma(-k, -0x1.718432a1b0e26p-35, r);
+    double p = 1.6059043836821613e-10;  // 1/13!
+    p = fma(p, r, 2.08767569878681e-09);
+    p = fma(p, r, 2.505210838544172e-08);
+    p = fma(p, r, 2.755731922398589e-07);
+    p = fma(p, r, 2.7557319223985893e-06);
+    p = fma(p, r, 2.48015873015873e-05);
+    p = fma(p, r, 1.984126984126984e-04);
+    p = fma(p, r, 1.388888888888889e-03);
+    p = fma(p, r, 8.333333333333333e-03);
+    p = fma(p, r, 4.1666666666666664e-02);
+    p = fma(p, r, 1.6666666666666666e-01);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(p, (int)k);
+}
+
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+// Omega (or Q) with the labels folded in, in the kernel's LDS image: for
+// 16-point chunk t, column block cb (of ncbp) and h, the 64 pairs
+// Opk[((t ncbp + cb) 2 + h) 64 + (L ^ 4(L/16))] = (y_j O[j][c], y_{j+4} O[j+4][c]),
+// j = 16t + 8h + L/16, c = 16cb + L%16, zero outside n x l; one thread per pair.
+__global__ __launch_bounds__(256) void pack_omega_f64_kernel(int n, int l, int nch, int ncbp,
+                                                             const double* __restrict__ O,
+                                                             int ldo,
+                                                             const double* __restrict__ yj,
+                                                             d2v* __restrict__ Opk) {
+    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (long)nch * ncbp * 128) return;
+    const int s = (int)(idx & 63), h = (int)((idx >> 6) & 1);
+    const long tc = idx >> 7;
+    const int cb = (int)(tc % ncbp), t = (int)(tc / ncbp);
+    const int L = s ^ ((s >> 4) << 2);
+    const int j = 16 * t + 8 * h + (L >> 4), c = 16 * cb + (L & 15);
+    d2v v = {0.0, 0.0};
+    if (c < l) {
+        if (j < n) v[0] = O[j + (size_t)c * ldo] * (yj ? yj[j] : 1.0);
+        if (j + 4 < n) v[1] = O[j + 4 + (size_t)c * ldo] * (yj ? yj[j + 4] : 1.0);
+    }
+    Opk[idx] = v;
+}
+
+template <int DK, int NCB, bool PK>
 __global__ __launch_bounds__(256) void rbf_project_f64_kernel(
     int m, int n, int d, int l, const double* __restrict__ Xi, int ldxi,
     const double* __restrict__ Xj, int ldxj, const double* __restrict__ ni,
     const double* __restrict__ nj, const double* __restrict__ yi, const double* __restrict__ yj,
-    double gamma, const double* __restrict__ O, int ldo, double beta, double* __restrict__ Y,
-    int ldy, int jps, long ystride) {
-    typedef double d2v __attribute__((ext_vector_type(2)));
+    double gamma, const double* __restrict__ O, int ldo, const d2v* __restrict__ Opk, int ncbp,
+    double beta, double* __restrict__ Y, int ldy, int jps, long ystride) {
     constexpr int JB = 16;
     {  // split z of the j range: points [z jps, min((z + 1) jps, n)), partial into Y + z ystride
         const int j0 = blockIdx.z * jps;
@@ -170,6 +229,7 @@ __global__ __launch_bounds__(256) void rbf_project_f64_kernel(
         nj += j0;
         if (yj) yj += j0;
         O += j0;
+        if (PK) Opk += (size_t)(j0 / JB) * ncbp * 128;
         Y += blockIdx.z * ystride;
     }
     constexpr int FD = DK * 4;  // padded feature count
@@ -200,7 +260,18 @@ __global__ __launch_bounds__(256) void rbf_project_f64_kernel(
     // block, so its label is one load per chunk.
     constexpr int NXL = JB * FD / 256, NOL = JB * NCB * 16 / 256;
     static_assert(JB * FD % 256 == 0 && JB * NCB * 16 % 256 == 0, "staging split");
-    double px[NXL], po[NOL], pn, yv;
+    double px[NXL], po[PK ? 1 : NOL], pn, yv;
+    // PK: the chunk's packed Omega image (NCB x 2 KB, contiguous) goes to LDS
+    // by direct copies, 1 KB per wave instruction.
+    auto dma_omega = [&](int j0, int b) {
+        const d2v* src = Opk + ((size_t)(j0 / JB) * ncbp + blockIdx.y * NCB) * 128;
+#pragma unroll
+        for (int k = 0; k < 2 * NCB; k += 4) {
+            const int kk = k + wave;
+            if ((2 * NCB) % 4 == 0 || kk < 2 * NCB)
+                dma16(src + kk * 64, (unsigned)lane * 16u, &sO[b][kk >> 1][kk & 1][0]);
+        }
+    };
     auto load = [&](int j0) {
 #pragma unroll
         for (int u = 0; u < NXL; ++u) {
@@ -208,11 +279,13 @@ __global__ __launch_bounds__(256) void rbf_project_f64_kernel(
             px[u] = Xj[min(f, d - 1) + (size_t)min(j0 + jj, n - 1) * ldxj];
         }
         const int jo = j0 + (tid % JB);
-        yv = yj ? yj[min(jo, n - 1)] : 1.0;
+        if constexpr (!PK) {
+            yv = yj ? yj[min(jo, n - 1)] : 1.0;
 #pragma unroll
-        for (int u = 0; u < NOL; ++u) {
-            const int e = tid + 256 * u, cl = e / JB;
-            po[u] = O[min(jo, n - 1) + (size_t)min(c0 + cl, l - 1) * ldo];
+            for (int u = 0; u < NOL; ++u) {
+                const int e = tid + 256 * u, cl = e / JB;
+                po[u] = O[min(jo, n - 1) + (size_t)min(c0 + cl, l - 1) * ldo];
+            }
         }
         pn = nj[min(j0 + (tid % JB), n - 1)];
     };
@@ -225,9 +298,9 @@ __global__ __launch_bounds__(256) void rbf_project_f64_kernel(
         }
         double* so = (double*)&sO[b][0][0][0];
 #pragma unroll
-        for (int u = 0; u < NOL; ++u) {
+        for (int u = 0; u < (PK ? 0 : NOL); ++u) {
             const int e = tid + 256 * u, jj = e % JB, cl = e / JB;
-            const int r = jj >> 2, ln = (jj & 3) * 16 + (cl & 15);
+            const int r = jj >> 2, ln = (jj & 3) * 16 + ((cl & 15) ^ ((jj & 3) << 2));
             so[(((cl >> 4) * 2 + (r >> 1)) * 64 + ln) * 2 + (r & 1)] =
                 (jo < n && c0 + cl < l) ? po[u] * yv : 0.0;
         }
@@ -235,12 +308,17 @@ __global__ __launch_bounds__(256) void rbf_project_f64_kernel(
     };
 
     load(0);
+    if constexpr (PK) dma_omega(0, 0);
     store(0, 0);
+    if constexpr (PK) wait_vm();
     __syncthreads();
     int b = 0;
     for (int j0 = 0; j0 < n; j0 += JB) {
         const bool more = j0 + JB < n;  // uniform
-        if (more) load(j0 + JB);
+        if (more) {
+            load(j0 + JB);
+            if constexpr (PK) dma_omega(j0 + JB, b ^ 1);
+        }
         dh_d4 s0 = dh_d4{0, 0, 0, 0}, s1 = dh_d4{0, 0, 0, 0};
         const double* xrow = &sX[b][li * XP + g * DK];
 #pragma unroll
@@ -254,17 +332,19 @@ __global__ __launch_bounds__(256) void rbf_project_f64_kernel(
         for (int r = 0; r < 4; ++r) {
             double d2 = nrow + sN[b][g + 4 * r] - 2.0 * (s0[r] + s1[r]);
             d2 = d2 > 0.0 ? d2 : 0.0;
-            kv[r] = exp(-gamma * d2);
+            kv[r] = exp_kernel_f64(-gamma * d2);
         }
+        const int sl = lane ^ ((lane >> 4) << 2);
 #pragma unroll
         for (int cb = 0; cb < NCB; ++cb) {
-            const d2v o01 = sO[b][cb][0][lane], o23 = sO[b][cb][1][lane];
+            const d2v o01 = sO[b][cb][0][sl], o23 = sO[b][cb][1][sl];
             acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(kv[0], o01[0], acc[cb], 0, 0, 0);
             acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(kv[1], o01[1], acc[cb], 0, 0, 0);
             acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(kv[2], o23[0], acc[cb], 0, 0, 0);
             acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(kv[3], o23[1], acc[cb], 0, 0, 0);
         }
         if (more) store(j0 + JB, b ^ 1);
+        if constexpr (PK) wait_vm();
         __syncthreads();
         b ^= 1;
     }
@@ -287,12 +367,18 @@ static void launch_project_f64(dh_handle_t h, dim3 grid, int m, int n, int d, in
                                const double* Xi, int ldxi, const double* Xj, int ldxj,
                                const double* ni, const double* nj, const double* yi,
                                const double* yj, double gamma, const double* O, int ldo,
-                               double beta, double* Y, int ldy, int jps, long ystride) {
+                               const d2v* Opk, int ncbp, double beta, double* Y, int ldy, int jps,
+                               long ystride) {
     if constexpr (DK < 64 || NCB <= 12) {  // d > 128 with 17 blocks is never selected (spills)
         dh_timing_begin(h);
-        rbf_project_f64_kernel<DK, NCB><<<grid, 256, 0, h->stream>>>(
-            m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, O, ldo, beta, Y, ldy, jps,
-            ystride);
+        if (Opk)
+            rbf_project_f64_kernel<DK, NCB, true><<<grid, 256, 0, h->stream>>>(
+                m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, O, ldo, Opk, ncbp, beta, Y,
+                ldy, jps, ystride);
+        else
+            rbf_project_f64_kernel<DK, NCB, false><<<grid, 256, 0, h->stream>>>(
+                m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, O, ldo, nullptr, 0, beta, Y,
+                ldy, jps, ystride);
         dh_timing_end(h);
     }
 }
@@ -301,12 +387,12 @@ template <int DK>
 static int dispatch_ncb(dh_handle_t h, int ncb, dim3 grid, int m, int n, int d, int l,
                         const double* Xi, int ldxi, const double* Xj, int ldxj, const double* ni,
                         const double* nj, const double* yi, const double* yj, double gamma,
-                        const double* O, int ldo, double beta, double* Y, int ldy, int jps,
-                        long ystride) {
+                        const double* O, int ldo, const d2v* Opk, int ncbp, double beta,
+                        double* Y, int ldy, int jps, long ystride) {
 #define DH_NCB(N)                                                                              \
     case N:                                                                                    \
         launch_project_f64<DK, N>(h, grid, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, \
-                                  O, ldo, beta, Y, ldy, jps, ystride);                         \
+                                  O, ldo, Opk, ncbp, beta, Y, ldy, jps, ystride);              \
         break;
     switch (ncb) {
         DH_NCB(1)
@@ -368,15 +454,30 @@ static int ncb_instance(int want) {
 
 // Y (m x l) = K(Xi, Xj) O + beta Y with the fused kernel, the j range in
 // `splits` parts (partials to P + z ystride, ld m, when splits > 1).
+// With `pack`, Omega is first packed (labels folded in) into the kernel's LDS
+// image in a workspace and staged by direct copies; without it (the
+// allocation-free entry point) the kernel stages it through registers.  Both
+// put the same values in LDS, so both give the same bits.
 static int project_f64_fused(dh_handle_t h, int m, int n, int d, int l, const double* Xi,
                              int ldxi, const double* Xj, int ldxj, const double* ni,
                              const double* nj, const double* yi, const double* yj, double gamma,
                              const double* O, int ldo, double beta, double* Y, int ldy, int ncb,
-                             int groups, int jps, int splits, long ystride) {
+                             int groups, int jps, int splits, long ystride, bool pack) {
     dim3 grid(dh_cdiv(m, 64), groups, splits);
+    const d2v* Opk = nullptr;
+    const int ncbp = groups * ncb, nch = dh_cdiv(n, 16);
+    if (pack) {
+        const long pairs = (long)nch * ncbp * 128;
+        d2v* P = (d2v*)dh_ws(h, DH_WS_PACK64, sizeof(d2v) * (size_t)pairs);
+        if (!P) return DH_ERR_ALLOC;
+        pack_omega_f64_kernel<<<dh_cdiv(pairs, 256), 256, 0, h->stream>>>(n, l, nch, ncbp, O, ldo,
+                                                                         yj, P);
+        DH_CHECK_LAUNCH();
+        Opk = P;
+    }
 #define DH_DK(K)                                                                              \
     dispatch_ncb<K>(h, ncb, grid, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, O, ldo, \
-                    beta, Y, ldy, jps, ystride)
+                    Opk, ncbp, beta, Y, ldy, jps, ystride)
     if (d <= 32) return DH_DK(8);
     if (d <= 64) return DH_DK(16);
     if (d <= 128) return DH_DK(32);
@@ -393,11 +494,12 @@ static void f64_columns(int d, int l, int* ncb, int* groups) {
     *groups = dh_cdiv(ncb_total, *ncb);
 }
 
-// Whole j range in one launch (no workspace for d <= 256).
+// Whole j range in one launch; `pack` as in project_f64_fused (without it, no
+// workspace for d <= 256).
 int dh_project_f64_impl(dh_handle_t h, int m, int n, int d, int l, const double* Xi, int ldxi,
                         const double* Xj, int ldxj, const double* ni, const double* nj,
                         const double* yi, const double* yj, double gamma, const double* O,
-                        int ldo, double beta, double* Y, int ldy) {
+                        int ldo, double beta, double* Y, int ldy, bool pack) {
     if (m == 0 || l == 0) return 0;
     if (n == 0)  // K has no columns: Y = beta Y
         return dh_gemm_impl<double>(h, false, false, m, l, 0, 1.0, O, 1, O, ldo, beta, Y, ldy, 0);
@@ -407,7 +509,7 @@ int dh_project_f64_impl(dh_handle_t h, int m, int n, int d, int l, const double*
     int ncb, groups;
     f64_columns(d, l, &ncb, &groups);
     return project_f64_fused(h, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, O, ldo,
-                             beta, Y, ldy, ncb, groups, n, 1, 0);
+                             beta, Y, ldy, ncb, groups, n, 1, 0, pack);
 }
 
 // Y = beta Y + sum over the splits z (in order) of P[z ystride + i + c m].
@@ -434,20 +536,20 @@ int dh_project_f64_split(dh_handle_t h, int m, int n, int d, int l, const double
                          int ldo, double beta, double* Y, int ldy) {
     if (m == 0 || l == 0 || n == 0 || d > 256)
         return dh_project_f64_impl(h, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, O,
-                                   ldo, beta, Y, ldy);
+                                   ldo, beta, Y, ldy, true);
     int ncb, groups;
     f64_columns(d, l, &ncb, &groups);
     const int nch = dh_cdiv(n, 32);
     const int splits = dh_j_splits(h->num_cu, (long)dh_cdiv(m, 64) * groups, nch);
     if (splits == 1)
         return project_f64_fused(h, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, O,
-                                 ldo, beta, Y, ldy, ncb, groups, n, 1, 0);
+                                 ldo, beta, Y, ldy, ncb, groups, n, 1, 0, true);
     const int jps = dh_cdiv(nch, splits) * 32;
     const long ystride = (long)m * l;
     double* P = (double*)dh_ws(h, DH_WS_SPLITN, sizeof(double) * (size_t)ystride * splits);
     if (!P) return DH_ERR_ALLOC;
     DH_CHECK_STATUS(project_f64_fused(h, m, n, d, l, Xi, ldxi, Xj, ldxj, ni, nj, yi, yj, gamma, O,
-                                      ldo, 0.0, P, m, ncb, groups, jps, splits, ystride));
+                                      ldo, 0.0, P, m, ncb, groups, jps, splits, ystride, true));
     sum_splits_f64_kernel<<<dh_cdiv(ystride, 256), 256, 0, h->stream>>>(m, l, splits, P, ystride,
                                                                         beta, Y, ldy);
     DH_CHECK_LAUNCH();
@@ -474,7 +576,7 @@ extern "C" int dh_rbf_project_f64(dh_handle_t h, int m, int n, int d, int l, con
     DH_CHECK_STATUS(launch_sqnorms<double>(h, m, d, Xi, ldxi, nrm));
     DH_CHECK_STATUS(launch_sqnorms<double>(h, n, d, Xj, ldxj, nrm + m));
     return dh_project_f64_impl(h, m, n, d, l, Xi, ldxi, Xj, ldxj, nrm, nrm + m, yi, yj, gamma, O,
-                               ldo, beta, Y, ldy);
+                               ldo, beta, Y, ldy, true);
 }
 
 // ------------------------------------------------------------ error pass
