@@ -32,11 +32,13 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall",
           "-Wno-unused-parameter", "-Wno-unused-function"]
 
-# Per-file device flags.  The fused FP64 pass keeps its MFMA accumulators in
-# VGPRs: left to the heuristics it allocated them in AGPRs and copied about 290
-# registers between the files every 16-point chunk (the allocator's choice
-# under a 510-register wave).
-FILE_FLAGS = {"rbf_f64.hip": ["-mllvm", "--amdgpu-mfma-vgpr-form=1"]}
+# Per-file device flags.  The FP64 / FP32 MFMA kernels keep their accumulators
+# in VGPRs: left to the heuristics, the fused FP64 pass allocated them in AGPRs
+# and copied about 290 registers between the register files every 16-point
+# chunk (a 510-register wave), and the GEMM tiles copied theirs too (the
+# 128 x 128 FP64 tile took 356 registers, 238 in VGPR form).
+_VGPR_MFMA = ["-mllvm", "--amdgpu-mfma-vgpr-form=1"]
+FILE_FLAGS = {f: _VGPR_MFMA for f in ("rbf_f64.hip", "dense.hip", "small_spd.hip", "batch.hip")}
 
 SOURCES = ["handle.hip", "dense.hip", "small_spd.hip", "solve.hip", "rbf_f64.hip", "rbf_bf16.hip",
            "rbf16_dk1.hip", "rbf16_dk2.hip", "rbf16_dk4.hip", "rbf16_dk8.hip",
