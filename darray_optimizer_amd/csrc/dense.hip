@@ -173,6 +173,14 @@ int dh_gemm_impl(dh_handle_t h, bool ta, bool tb, int m, int n, int k, T alpha, 
     return gemm_stop<T>(h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, uplo_upper,
                         nullptr);
 }
+template <typename T>
+int dh_gemm_stop_impl(dh_handle_t h, bool ta, bool tb, int m, int n, int k, T alpha, const T* A,
+                      int lda, const T* B, int ldb, T beta, T* C, int ldc, const int* stop) {
+    return gemm_stop<T>(h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 0, stop);
+}
+template int dh_gemm_stop_impl<double>(dh_handle_t, bool, bool, int, int, int, double,
+                                       const double*, int, const double*, int, double, double*,
+                                       int, const int*);
 template int dh_gemm_impl<double>(dh_handle_t, bool, bool, int, int, int, double, const double*,
                                   int, const double*, int, double, double*, int, int);
 template int dh_gemm_impl<float>(dh_handle_t, bool, bool, int, int, int, float, const float*,

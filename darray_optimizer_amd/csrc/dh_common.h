@@ -32,6 +32,7 @@ enum dh_slot {
     DH_WS_KBLOCK,      // materialised K row block of the d > 256 FP64 pass
     DH_WS_SPLITN,      // partial products of a K-pass whose j range is split
     DH_WS_PACK64,      // packed FP64 Omega / Q of the fused FP64 pass
+    DH_WS_GUARD,       // combined status word that stops a GEMM after a breakdown
     DH_WS_COUNT
 };
 
@@ -133,6 +134,10 @@ static inline int dh_j_splits(int num_cu, long workgroups, int nchunks) {
 template <typename T>
 int dh_gemm_impl(dh_handle_t h, bool ta, bool tb, int m, int n, int k, T alpha, const T* A,
                  int lda, const T* B, int ldb, T beta, T* C, int ldc, int uplo_upper);
+// As dh_gemm_impl (no triangle mask), doing nothing when *stop != 0 on the device.
+template <typename T>
+int dh_gemm_stop_impl(dh_handle_t h, bool ta, bool tb, int m, int n, int k, T alpha, const T* A,
+                      int lda, const T* B, int ldb, T beta, T* C, int ldc, const int* stop);
 template <typename T>
 int dh_trsm_impl(dh_handle_t h, char side, char trans, bool unit, int m, int n, T alpha,
                  const T* A, int lda, T* B, int ldb);

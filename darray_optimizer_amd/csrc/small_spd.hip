@@ -70,6 +70,11 @@ int dh_cholqr_step_batched(dh_handle_t h, const DhSpdJob* jobs, int count, int m
     return 0;
 }
 
+// *w = guard[0] | guard[1] (nguard words): one stop word for a GEMM.
+__global__ void combine_guard_kernel(const int* __restrict__ guard, int nguard, int* __restrict__ w) {
+    if (threadIdx.x == 0) *w = guard[0] | (nguard > 1 ? guard[1] : 0);
+}
+
 // Y := Y U^{-1} with U from chol_small (upper, order n <= 544) and its Dinv.
 // With guard != nullptr the kernel does nothing unless guard[0 .. nguard-1]
 // (device status words of the Cholesky steps before it, nguard <= 2) are
@@ -82,6 +87,34 @@ int dh_trsm_rows_launch(dh_handle_t h, int m, int n, double* Y, int ldy, const d
         constexpr int WR = 4;
         trsm_rows_reg_kernel<WR><<<dh_cdiv(m, 16 * WR), WR * 64, 0, h->stream>>>(m, n, Y, ldy, U,
                                                                                ldu, Dinv, guard, nguard);
+        DH_CHECK_LAUNCH();
+        return 0;
+    }
+    {
+        // 272 < n <= 544: U = [U11 U12; 0 U22] with U11 of order 272.  X1 = Y1 U11^-1
+        // and X2 = (Y2 - X1 U12) U22^-1 by the register row solve around one FP64
+        // GEMM (the single-kernel row solve below keeps the whole panel in LDS and
+        // ran at ~13 TFLOP/s at n = 262144, l = 544).  The GEMM is skipped after
+        // a breakdown, as the solves are, through one combined status word.
+        constexpr int WR = 4;
+        const int n1 = 16 * kRegBlocks, n2 = n - n1;
+        trsm_rows_reg_kernel<WR><<<dh_cdiv(m, 16 * WR), WR * 64, 0, h->stream>>>(
+            m, n1, Y, ldy, U, ldu, Dinv, guard, nguard);
+        DH_CHECK_LAUNCH();
+        const int* stop = nullptr;
+        if (guard) {
+            int* w = (int*)dh_ws(h, DH_WS_GUARD, sizeof(int));
+            if (!w) return DH_ERR_ALLOC;
+            combine_guard_kernel<<<1, 64, 0, h->stream>>>(guard, nguard, w);
+            DH_CHECK_LAUNCH();
+            stop = w;
+        }
+        DH_CHECK_STATUS(dh_gemm_stop_impl<double>(h, false, false, m, n2, n1, -1.0, Y, ldy,
+                                                  U + (size_t)n1 * ldu, ldu, 1.0,
+                                                  Y + (size_t)n1 * ldy, ldy, stop));
+        trsm_rows_reg_kernel<WR><<<dh_cdiv(m, 16 * WR), WR * 64, 0, h->stream>>>(
+            m, n2, Y + (size_t)n1 * ldy, ldy, U + n1 + (size_t)n1 * ldu, ldu,
+            Dinv + 256 * kRegBlocks, guard, nguard);
         DH_CHECK_LAUNCH();
         return 0;
     }

@@ -298,10 +298,13 @@ def test_cholqr2_golden(dev):
 
 
 @pytest.mark.parametrize("m,n", [(20000, 138), (3000, 1), (3000, 257), (5000, 272), (6000, 544),
-                                 (4000, 600), (131, 130), (4099, 200), (70001, 129)])
+                                 (5000, 300), (2503, 417), (4000, 600), (131, 130), (4099, 200),
+                                 (70001, 129)])
 def test_cholqr2_tall(dev, m, n):
     # n <= 272: register-resident Cholesky (257: one real column in the last
-    # 16-block); n <= 544: one-workgroup Cholesky; n = 600: blocked fallback.
+    # 16-block); n <= 544: one-workgroup Cholesky and the row solve as two
+    # register solves around a GEMM (300, 417: ragged second block); n = 600:
+    # blocked fallback.
     # 128 < n <= 272 also takes the register-blocked Gram: (131, 130) one slab
     # per 16 rows with a ragged last chunk, (70001, 129) ragged row slabs.
     import torch
