@@ -473,7 +473,9 @@ def main(argv=None) -> int:
     fp64["roofline"] = {"bound": "mfma", "achieved": pass_flops / t_f64 / 1e12,
                         "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s",
                         "frac": pass_flops / t_f64 / PEAK_FP64_MFMA,
-                        "kernel": "rbf_project_f64_kernel<32,17>", "kernel_ms": t_f64 * 1e3}
+                        "traffic": measured_traffic(
+                            ROOT / "profiles" / "rbf_project_f64_n65536_d128_l272.md"),
+                        "kernel": "rbf_project_f64_kernel<32,17,packed>", "kernel_ms": t_f64 * 1e3}
 
     line = {
         "metric": "RBF-LRA GFLOP/s & s/solve, mixed vs FP64 on 1 MI355X at matched ||K-QB||/||K||",
