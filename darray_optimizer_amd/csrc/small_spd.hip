@@ -84,7 +84,7 @@ int dh_trsm_rows_launch(dh_handle_t h, int m, int n, double* Y, int ldy, const d
     if (n > kMaxN) return dh_fail(DH_ERR_UNSUPPORTED, "trsm_rows: order %d > %d", n, kMaxN);
     if (m == 0 || n == 0) return 0;
     if (n <= 16 * kRegBlocks) {
-        constexpr int WR = 4;
+        constexpr int WR = 8;
         trsm_rows_reg_kernel<WR><<<dh_cdiv(m, 16 * WR), WR * 64, 0, h->stream>>>(m, n, Y, ldy, U,
                                                                                ldu, Dinv, guard, nguard);
         DH_CHECK_LAUNCH();
@@ -96,7 +96,7 @@ int dh_trsm_rows_launch(dh_handle_t h, int m, int n, double* Y, int ldy, const d
         // GEMM (the single-kernel row solve below keeps the whole panel in LDS and
         // ran at ~13 TFLOP/s at n = 262144, l = 544).  The GEMM is skipped after
         // a breakdown, as the solves are, through one combined status word.
-        constexpr int WR = 4;
+        constexpr int WR = 8;
         const int n1 = 16 * kRegBlocks, n2 = n - n1;
         trsm_rows_reg_kernel<WR><<<dh_cdiv(m, 16 * WR), WR * 64, 0, h->stream>>>(
             m, n1, Y, ldy, U, ldu, Dinv, guard, nguard);
