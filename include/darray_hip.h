@@ -100,11 +100,12 @@ int dh_rbf_tile_f32(dh_handle_t h, int m, int n, int d, const float* Xi, int ldx
                     float* K, int ldk);
 
 /* Fused Y[m x l] = K(Xi,Xj) * O[n x l] + beta*Y in FP64 (MFMA f64 for the distance
- * and the projection, FP64 exp).  K is never stored for d <= 256; above 256
- * features K is materialised in row blocks of at most 2^27 entries (a 1 GiB
- * workspace), as rbf_kernel does (dkernel.cu:144-178), and multiplied into O by
- * the FP64 GEMM.  Replaces the rbf_kernel + replicate + tc_gemm sequence of
- * DArray::LRA (dkernel.h:33-42). */
+ * and the projection, FP64 exp).  K is never stored for d <= 256 (O, with yj folded
+ * in, is first packed into a workspace of 8 * ceil(n/16) * 16 * 16 * ceil(l/16)
+ * bytes in the kernel's operand order); above 256 features K is materialised in
+ * row blocks of at most 2^27 entries (a 1 GiB workspace), as rbf_kernel does
+ * (dkernel.cu:144-178), and multiplied into O by the FP64 GEMM.  Replaces the
+ * rbf_kernel + replicate + tc_gemm sequence of DArray::LRA (dkernel.h:33-42). */
 int dh_rbf_project_f64(dh_handle_t h, int m, int n, int d, int l, const double* Xi, int ldxi,
                        const double* Xj, int ldxj, const double* yi, const double* yj,
                        double gamma, const double* O, int ldo, double beta, double* Y, int ldy);
