@@ -35,9 +35,7 @@ constexpr int kTileBK = 16;
 constexpr int kTilePad = 4;
 
 // Accumulate op(A)[m0:m0+BM, kbeg:kend] * op(B)[kbeg:kend, n0:n0+BN] into acc
-// with a 256-thread workgroup (or each 256-thread half of a 512-thread one,
-// the halves with their own sA / sB and the same k range, so that they meet
-// the same barriers): waves in a 2 x 2 grid, each owning a
+// with a 256-thread workgroup: waves in a 2 x 2 grid, each owning a
 // (BM/2) x (BN/2) sub-tile as (BM/32) x (BN/32) MFMA blocks.  K advances in
 // steps of 16 staged through LDS as op(A)^T and op(B) (k-major rows), so each
 // MFMA operand read is 16 consecutive LDS elements; the global loads of step
@@ -55,7 +53,7 @@ __device__ __forceinline__ void tile_mma(typename Mfma<T>::acc_t (&acc)[BM / 32]
     constexpr int BK = kTileBK;
     constexpr int FM = BM / 32, FN = BN / 32;
     constexpr int PA = BM * BK / 256, PB = BN * BK / 256;  // staged elements per thread
-    const int tid = threadIdx.x & 255, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int g = lane >> 4, li = lane & 15;
     T ra[PA], rb[PB];
@@ -132,7 +130,7 @@ __device__ __forceinline__ int tile_row(int a, int r) {
 }
 template <int BM, int BN>
 __device__ __forceinline__ int tile_col(int b) {
-    const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) & 3;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     return (wave >> 1) * (BN / 2) + b * 16 + (lane & 15);
 }
 
