@@ -7,8 +7,9 @@
 //   per pass : pack the operand (Omega or Q) -> persistent 16-bit projection
 //              kernels, one per tile class (dpad, column blocks), pulling
 //              (problem, row block) items from an atomic queue, longest
-//              problem first -> CholeskyQR2 (batched split-K Gram, one
-//              workgroup Cholesky per problem, row-parallel solve), twice
+//              problem first -> CholeskyQR2 (register-blocked Gram over the
+//              row slabs of every problem, one workgroup Cholesky per
+//              problem, row-parallel solve), twice
 //   copies   : W -> Q after each power iteration
 // and the sequence is captured once in a hipGraph and replayed while the
 // problem table is unchanged.  Queue counters are cleared by a fill kernel
@@ -30,66 +31,13 @@
 #include <vector>
 
 #include "batch.h"
-#include "mfma_tile.h"
 
 namespace {
 
 constexpr float kLog2e = 1.4426950408889634f;
-constexpr int kGramK = 4096;  // rows of Y per split-K Gram workgroup
 #define DH_BF16_DEFAULT_WIDE 0      // schedule of a class when tuning is off
 
 // ------------------------------------------------------------------ kernels
-
-struct GramJob {
-    int m, n, ldy, nsplit;
-    const double* Y;
-    double* part;  // nsplit x (n x n)
-    double* G;     // n x n, ld n
-};
-
-// Upper 64 x 64 tiles of Y^T Y over a k-range: blockIdx.x = upper tile,
-// blockIdx.y = split, blockIdx.z = job.
-__global__ __launch_bounds__(256) void gram_batched_kernel(const GramJob* __restrict__ jobs) {
-    const GramJob& j = jobs[blockIdx.z];
-    const int nt = (j.n + 63) / 64;
-    if ((int)blockIdx.x >= nt * (nt + 1) / 2 || (int)blockIdx.y >= j.nsplit) return;
-    int bi = 0, rem = blockIdx.x;
-    while (rem >= nt - bi) {
-        rem -= nt - bi;
-        ++bi;
-    }
-    const int bj = bi + rem;
-    __shared__ double sA[kTileBK][64 + kTilePad], sB[kTileBK][64 + kTilePad];
-    dh_d4 acc[2][2];
-    zero_acc<double, 2, 2>(acc);
-    const int kbeg = blockIdx.y * kGramK, kend = min(j.m, kbeg + kGramK);
-    tile_mma<double, 64, 64, true, false>(acc, sA, sB, j.n, j.n, 64 * bi, 64 * bj, kbeg, kend, j.Y,
-                                          j.ldy, j.Y, j.ldy);
-    double* P = j.part + (size_t)blockIdx.y * j.n * j.n;
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                int i = 64 * bi + tile_row<double, 64, 64>(a, r);
-                int c = 64 * bj + tile_col<64, 64>(b);
-                if (i < j.n && c < j.n) P[i + (size_t)c * j.n] = acc[a][b][r];
-            }
-}
-
-// G = sum of the split partials on the upper triangle (fixed order: deterministic).
-__global__ __launch_bounds__(256) void gram_reduce_batched_kernel(const GramJob* __restrict__ jobs) {
-    const GramJob& j = jobs[blockIdx.y];
-    const size_t nn = (size_t)j.n * j.n;
-    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < nn; e += (size_t)gridDim.x * 256) {
-        int i = (int)(e % j.n), c = (int)(e / j.n);
-        if (i > c) continue;
-        double s = 0.0;
-        for (int t = 0; t < j.nsplit; ++t) s += j.part[t * nn + e];
-        j.G[e] = s;
-    }
-}
 
 struct CopyJob {
     int m, n, lds, ldd;
@@ -383,7 +331,7 @@ int build_plan(dh_handle_t h, Plan& P, int prec, int count, const dh_lra_problem
 
     // ---- sizes and layout
     struct Inst {
-        int dpad, npad, nchunks, ncb, groups, ncb_pack, rows, nsplit;
+        int dpad, npad, nchunks, ncb, groups, ncb_pack, rows, nslab;
         size_t o_part, o_mean, o_xb, o_a, o_op, o_g, o_gpart, o_dinv;
     };
     std::vector<Inst> in(count);
@@ -391,13 +339,19 @@ int build_plan(dh_handle_t h, Plan& P, int prec, int count, const dh_lra_problem
     const size_t o_sets = cv.take(sizeof(DhBf16Set) * count);
     const size_t o_pack = cv.take(sizeof(DhPackJob) * count * 2);
     const size_t o_proj = cv.take(sizeof(DhProjJob) * count * 2);
-    const size_t o_gram = cv.take(sizeof(GramJob) * count * 2);
+    const size_t o_gram = cv.take(sizeof(DhGramRegJob) * count * 2);
     const size_t o_spd = cv.take(sizeof(DhSpdJob) * count * 2);
     const size_t o_copy = cv.take(sizeof(CopyJob) * count);
     const size_t o_info = cv.take(sizeof(int) * count * 4);  // statuses, sticky, FP16 maxabs
-    int max_n = 0, max_d = 0, max_npad = 0, max_l = 0, max_tri = 0, max_split = 0;
-    long max_pack_threads = 0, max_nn = 0, max_nl = 0;
+    int max_n = 0, max_d = 0, max_npad = 0, max_l = 0;
+    long max_pack_threads = 0, max_nl = 0;
     size_t nitems = 0;
+    // Gram slabs: about four register-blocked workgroups per CU over the batch.
+    long total_rows = 0;
+    for (int i = 0; i < count; ++i) total_rows += probs[i].n;
+    const int gram_rows =
+        std::max(256, 16 * (int)dh_cdiv(dh_cdiv(total_rows, 4L * h->num_cu), 16L));
+    int ngitems = 0;
     for (int i = 0; i < count; ++i) {
         const dh_lra_problem& p = probs[i];
         Inst& s = in[i];
@@ -406,28 +360,26 @@ int build_plan(dh_handle_t h, Plan& P, int prec, int count, const dh_lra_problem
         s.nchunks = dh_cdiv(p.n, 32);
         dh_bf16_columns(p.l, &s.ncb, &s.groups, &s.ncb_pack);
         s.rows = dh_cdiv(p.n, dh_bf16_rows(s.dpad));
-        s.nsplit = dh_cdiv(p.n, kGramK);
+        s.nslab = dh_cdiv(p.n, gram_rows);
+        ngitems += s.nslab;
         s.o_part = cv.take(sizeof(double) * (size_t)dh_cdiv(p.n, 256) * p.d);
         s.o_mean = cv.take(sizeof(double) * p.d);
         s.o_xb = cv.take((size_t)s.npad * s.dpad * 2);
         s.o_a = cv.take(sizeof(float) * s.npad);
         s.o_op = cv.take(dh_bf16_packed_bytes(p.n, s.ncb_pack));
         s.o_g = cv.take(sizeof(double) * (size_t)p.l * p.l);
-        s.o_gpart = cv.take(sizeof(double) * (size_t)s.nsplit * p.l * p.l);
+        s.o_gpart = cv.take(sizeof(double) * (size_t)s.nslab * dh_gram_reg_slab_doubles());
         s.o_dinv = cv.take(sizeof(double) * 256 * (size_t)dh_cdiv(p.l, 16));
         nitems += (size_t)s.rows * s.groups;
         max_n = std::max(max_n, p.n);
         max_d = std::max(max_d, p.d);
         max_npad = std::max(max_npad, s.npad);
         max_l = std::max(max_l, p.l);
-        int nt = dh_cdiv(p.l, 64);
-        max_tri = std::max(max_tri, nt * (nt + 1) / 2);
-        max_split = std::max(max_split, s.nsplit);
         max_pack_threads = std::max(max_pack_threads, (long)s.nchunks * s.ncb_pack * 16);
-        max_nn = std::max(max_nn, (long)p.l * p.l);
         max_nl = std::max(max_nl, (long)p.n * p.l);
     }
     const size_t o_items = cv.take(sizeof(DhProjItem) * nitems);
+    const size_t o_gitems = cv.take(sizeof(DhGramItem) * ngitems);
 
     // ---- tile classes and the longest-first work list
     std::vector<DhProjItem> items;
@@ -462,7 +414,7 @@ int build_plan(dh_handle_t h, Plan& P, int prec, int count, const dh_lra_problem
     std::vector<DhBf16Set> sets(count);
     std::vector<DhPackJob> pack(2 * count);
     std::vector<DhProjJob> proj(2 * count);
-    std::vector<GramJob> gram(2 * count);
+    std::vector<DhGramRegJob> gram(2 * count);
     std::vector<DhSpdJob> spd(2 * count);
     std::vector<CopyJob> copy(count);
     P.infos = (int*)at(o_info);
@@ -483,8 +435,8 @@ int build_plan(dh_handle_t h, Plan& P, int prec, int count, const dh_lra_problem
                                    t == 0 ? p.Q : p.W, t == 0 ? p.ldq : p.ldw};
             double* Y = t == 0 ? p.Q : p.W;
             int ldy = t == 0 ? p.ldq : p.ldw;
-            gram[t * count + i] = {p.n, p.l, ldy, s.nsplit, Y, (double*)at(s.o_gpart),
-                                   (double*)at(s.o_g)};
+            gram[t * count + i] = {p.n, p.l, ldy, gram_rows, s.nslab, Y,
+                                   (double*)at(s.o_gpart), (double*)at(s.o_g), p.l};
             spd[t * count + i] = {p.n, p.l, ldy, p.l, Y, (double*)at(s.o_g), (double*)at(s.o_dinv),
                                   P.infos + t * count + i, P.sticky + i};
         }
@@ -496,7 +448,18 @@ int build_plan(dh_handle_t h, Plan& P, int prec, int count, const dh_lra_problem
     DH_CHECK_HIP(up(o_sets, sets.data(), sizeof(DhBf16Set) * count));
     DH_CHECK_HIP(up(o_pack, pack.data(), sizeof(DhPackJob) * 2 * count));
     DH_CHECK_HIP(up(o_proj, proj.data(), sizeof(DhProjJob) * 2 * count));
-    DH_CHECK_HIP(up(o_gram, gram.data(), sizeof(GramJob) * 2 * count));
+    DH_CHECK_HIP(up(o_gram, gram.data(), sizeof(DhGramRegJob) * 2 * count));
+    // Gram items, largest orders first (their slabs cost the most MFMAs).
+    std::vector<int> gorder(count);
+    for (int i = 0; i < count; ++i) gorder[i] = i;
+    std::stable_sort(gorder.begin(), gorder.end(), [&](int a, int b) {
+        return probs[a].l > probs[b].l;
+    });
+    std::vector<DhGramItem> gitems;
+    gitems.reserve(ngitems);
+    for (int i : gorder)
+        for (int z = 0; z < in[i].nslab; ++z) gitems.push_back({i, z});
+    DH_CHECK_HIP(up(o_gitems, gitems.data(), sizeof(DhGramItem) * gitems.size()));
     DH_CHECK_HIP(up(o_spd, spd.data(), sizeof(DhSpdJob) * 2 * count));
     DH_CHECK_HIP(up(o_copy, copy.data(), sizeof(CopyJob) * count));
     DH_CHECK_HIP(up(o_items, items.data(), sizeof(DhProjItem) * items.size()));
@@ -506,7 +469,8 @@ int build_plan(dh_handle_t h, Plan& P, int prec, int count, const dh_lra_problem
     const DhPackJob* d_pack = (const DhPackJob*)at(o_pack);
     DhProjJob* d_proj = (DhProjJob*)at(o_proj);
     const bool f16 = prec == DH_PREC_F16;
-    const GramJob* d_gram = (const GramJob*)at(o_gram);
+    const DhGramRegJob* d_gram = (const DhGramRegJob*)at(o_gram);
+    const DhGramItem* d_gitems = (const DhGramItem*)at(o_gitems);
     const DhSpdJob* d_spd = (const DhSpdJob*)at(o_spd);
     const CopyJob* d_copy = (const CopyJob*)at(o_copy);
     const DhProjItem* d_items = (const DhProjItem*)at(o_items);
@@ -523,13 +487,10 @@ int build_plan(dh_handle_t h, Plan& P, int prec, int count, const dh_lra_problem
 
     // ---- capture
     auto orth = [&](int t) -> int {
-        const GramJob* gj = d_gram + t * count;
+        const DhGramRegJob* gj = d_gram + t * count;
         const DhSpdJob* sj = d_spd + t * count;
         for (int sweep = 0; sweep < 2; ++sweep) {
-            gram_batched_kernel<<<dim3(max_tri, max_split, count), 256, 0, h->stream>>>(gj);
-            DH_CHECK_LAUNCH();
-            gram_reduce_batched_kernel<<<dim3(dh_cdiv(max_nn, 256), count), 256, 0, h->stream>>>(gj);
-            DH_CHECK_LAUNCH();
+            DH_CHECK_STATUS(dh_gram_reg_batched(h, gj, count, d_gitems, ngitems));
             DH_CHECK_STATUS(dh_cholqr_step_batched(h, sj, count, max_l, max_n, false));
         }
         return 0;

@@ -161,3 +161,21 @@ struct DhSpdJob {
 };
 int dh_cholqr_step_batched(dh_handle_t h, const DhSpdJob* jobs, int count, int max_n, int max_m,
                            bool shift);
+
+// Register-blocked CholeskyQR Gram of one problem of a batch (order n <= 272):
+// G = upper(Y^T Y) from nslab slabs of `rows` rows each, slab partials in part
+// (dh_gram_reg_slab_doubles() doubles per slab), summed in slab order.
+struct DhGramRegJob {
+    int m, n, ldy, rows, nslab;
+    const double* Y;
+    double* part;
+    double* G;
+    int ldg;
+};
+struct DhGramItem {
+    int job, slab;
+};
+size_t dh_gram_reg_slab_doubles();
+// One launch over every (problem, slab) item, then one fixed-order reduction.
+int dh_gram_reg_batched(dh_handle_t h, const DhGramRegJob* jobs, int count,
+                        const DhGramItem* items, int nitems);

@@ -902,7 +902,7 @@ __device__ __forceinline__ void gram_mfma(dh_d4 (&acc)[kGramSlots],
 
 template <int W>
 __device__ __forceinline__ void gram_body(int m, int n, const double* __restrict__ Y, int ldy,
-                                          int r0, int r1, double* __restrict__ part,
+                                          int r0, int r1, double* __restrict__ pz,
                                           double (*sY)[16][kGramPitch]) {
     const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, li = lane & 15;
     const int nbk = (n + 15) / 16;
@@ -945,13 +945,27 @@ __device__ __forceinline__ void gram_body(int m, int n, const double* __restrict
         __syncthreads();
         buf ^= 1;
     }
-    double* pz = part + (size_t)blockIdx.x * kGramBlocks * 256;
+    // Blocks past the order (bj >= nbk) are neither written nor read back.
 #pragma unroll
     for (int s = 0; s < kGramSlots; ++s) {
         const int b = s * kCholWaves + W;
-        if (b < kGramBlocks)
+        if (b < kGramBlocks && kGramMap.bj[b] < nbk)
 #pragma unroll
             for (int r = 0; r < 4; ++r) pz[(size_t)b * 256 + r * 64 + lane] = acc[s][r];
+    }
+}
+
+template <typename F>
+__device__ __forceinline__ void gram_by_wave(F&& body) {
+    switch (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) {
+        case 0: body(std::integral_constant<int, 0>{}); break;
+        case 1: body(std::integral_constant<int, 1>{}); break;
+        case 2: body(std::integral_constant<int, 2>{}); break;
+        case 3: body(std::integral_constant<int, 3>{}); break;
+        case 4: body(std::integral_constant<int, 4>{}); break;
+        case 5: body(std::integral_constant<int, 5>{}); break;
+        case 6: body(std::integral_constant<int, 6>{}); break;
+        default: body(std::integral_constant<int, 7>{}); break;
     }
 }
 
@@ -959,23 +973,26 @@ __global__ __launch_bounds__(NT) void gram_reg_kernel(int m, int n, const double
                                                       int ldy, int rows, double* __restrict__ part) {
     __shared__ double sY[2][16][kGramPitch];
     const int r0 = blockIdx.x * rows, r1 = min(m, r0 + rows);
-    switch (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) {
-        case 0: gram_body<0>(m, n, Y, ldy, r0, r1, part, sY); break;
-        case 1: gram_body<1>(m, n, Y, ldy, r0, r1, part, sY); break;
-        case 2: gram_body<2>(m, n, Y, ldy, r0, r1, part, sY); break;
-        case 3: gram_body<3>(m, n, Y, ldy, r0, r1, part, sY); break;
-        case 4: gram_body<4>(m, n, Y, ldy, r0, r1, part, sY); break;
-        case 5: gram_body<5>(m, n, Y, ldy, r0, r1, part, sY); break;
-        case 6: gram_body<6>(m, n, Y, ldy, r0, r1, part, sY); break;
-        default: gram_body<7>(m, n, Y, ldy, r0, r1, part, sY); break;
-    }
+    double* pz = part + (size_t)blockIdx.x * kGramBlocks * 256;
+    gram_by_wave([&](auto w) { gram_body<decltype(w)::value>(m, n, Y, ldy, r0, r1, pz, sY); });
+}
+
+// Batched: one workgroup per (problem, slab) item, the batch's problems in
+// one launch; the slabs of a problem are reduced in order by the kernel below.
+__global__ __launch_bounds__(NT) void gram_reg_batched_kernel(const DhGramRegJob* __restrict__ jobs,
+                                                              const DhGramItem* __restrict__ items) {
+    __shared__ double sY[2][16][kGramPitch];
+    const DhGramItem it = items[blockIdx.x];
+    const DhGramRegJob& j = jobs[it.job];
+    const int r0 = it.slab * j.rows, r1 = min(j.m, r0 + j.rows);
+    double* pz = j.part + (size_t)it.slab * kGramBlocks * 256;
+    gram_by_wave([&](auto w) { gram_body<decltype(w)::value>(j.m, j.n, j.Y, j.ldy, r0, r1, pz, sY); });
 }
 
 // G(i, j), i <= j < n, = sum over the nz slabs of part: 64 elements per
 // workgroup, four thread groups over interleaved slabs, then an LDS sum.
-__global__ __launch_bounds__(256) void gram_reduce_kernel(int n, int nz,
-                                                          const double* __restrict__ part,
-                                                          double* __restrict__ G, int ldg) {
+__device__ __forceinline__ void gram_reduce_body(int n, int nz, const double* __restrict__ part,
+                                                 double* __restrict__ G, int ldg) {
     __shared__ double red[4][64];
     const int e = blockIdx.x * 64 + (threadIdx.x & 63), zg = threadIdx.x >> 6;
     double s0 = 0.0, s1 = 0.0;
@@ -998,6 +1015,20 @@ __global__ __launch_bounds__(256) void gram_reduce_kernel(int n, int nz,
         const int col = 16 * kGramMap.bj[b] + (ln & 15);
         if (col < n && row <= col) G[row + (size_t)col * ldg] = t;
     }
+}
+
+__global__ __launch_bounds__(256) void gram_reduce_kernel(int n, int nz,
+                                                          const double* __restrict__ part,
+                                                          double* __restrict__ G, int ldg) {
+    gram_reduce_body(n, nz, part, G, ldg);
+}
+
+// blockIdx.y = problem; workgroups of blocks past its order leave at once.
+__global__ __launch_bounds__(256) void gram_reduce_batched_kernel(
+    const DhGramRegJob* __restrict__ jobs) {
+    const DhGramRegJob& j = jobs[blockIdx.y];
+    if (16 * kGramMap.bj[min((int)blockIdx.x * 64 / 256, kGramBlocks - 1)] >= j.n) return;
+    gram_reduce_body(j.n, j.nslab, j.part, j.G, j.ldg);
 }
 
 }  // namespace

@@ -21,6 +21,19 @@ int dh_gram_small_launch(dh_handle_t h, int m, int n, const double* Y, int ldy, 
     return 0;
 }
 
+size_t dh_gram_reg_slab_doubles() { return (size_t)kGramBlocks * 256; }
+
+int dh_gram_reg_batched(dh_handle_t h, const DhGramRegJob* jobs, int count,
+                        const DhGramItem* items, int nitems) {
+    if (count == 0 || nitems == 0) return 0;
+    gram_reg_batched_kernel<<<nitems, NT, 0, h->stream>>>(jobs, items);
+    DH_CHECK_LAUNCH();
+    gram_reduce_batched_kernel<<<dim3(dh_cdiv(kGramBlocks * 256, 64), count), 256, 0, h->stream>>>(
+        jobs);
+    DH_CHECK_LAUNCH();
+    return 0;
+}
+
 // One-workgroup Cholesky of G (+ 16 x 16 diagonal-block inverses into Dinv,
 // which needs 256 * ceil(n/16) doubles); *info_dev receives the status.
 int dh_chol_small_launch(dh_handle_t h, int n, double* G, int ldg, double* Dinv, int m_rows,
