@@ -33,6 +33,7 @@ enum dh_slot {
     DH_WS_SPLITN,      // partial products of a K-pass whose j range is split
     DH_WS_PACK64,      // packed FP64 Omega / Q of the fused FP64 pass
     DH_WS_GUARD,       // combined status word that stops a GEMM after a breakdown
+    DH_WS_CHOL2,       // scales and block statuses of the two-block small Cholesky
     DH_WS_COUNT
 };
 

@@ -615,9 +615,13 @@ struct CholRegLds {
     int s_info;
 };
 
+// scales (nullable): {tol, shift} computed from a larger matrix whose trailing
+// block G is (the Schur complement step of an order-544 factorisation); then
+// they replace the ones from G's own diagonal.
 __device__ __forceinline__ void chol_reg_body(CholRegLds& L, int n, double* __restrict__ G,
                                               int ldg, double* __restrict__ Dinv, int m_rows,
-                                              int shift, int rel_check, int* __restrict__ info) {
+                                              int shift, int rel_check, int* __restrict__ info,
+                                              const double* __restrict__ scales = nullptr) {
     auto& sG = L.sG;
     auto& sI = L.sI;
     auto& sP = L.sP;
@@ -784,9 +788,13 @@ __device__ __forceinline__ void chol_reg_body(CholRegLds& L, int n, double* __re
             t2 += red[1][w];
         }
         tol = rel_check ? (double)n * 2.220446049250313e-16 * m2 : 0.0;
-        const double sh = shift ? 11.0 * ((double)m_rows * n + (double)n * (n + 1)) *
-                                      1.1102230246251565e-16 * t2
-                                : 0.0;
+        double sh = shift ? 11.0 * ((double)m_rows * n + (double)n * (n + 1)) *
+                                1.1102230246251565e-16 * t2
+                          : 0.0;
+        if (scales) {
+            tol = scales[0];
+            sh = scales[1];
+        }
         if (shift && tid < n) sG[tid >> 4][tid & 15][tid & 15] += sh;
         if (tid == 0) s_info = 0;
     }
@@ -842,9 +850,31 @@ __device__ __forceinline__ void chol_reg_body(CholRegLds& L, int n, double* __re
 __global__ __launch_bounds__(NT) void chol_reg_kernel(int n, double* __restrict__ G, int ldg,
                                                       double* __restrict__ Dinv, int m_rows,
                                                       int shift, int rel_check,
-                                                      int* __restrict__ info) {
+                                                      int* __restrict__ info,
+                                                      const double* __restrict__ scales) {
     __shared__ CholRegLds lds;
-    chol_reg_body(lds, n, G, ldg, Dinv, m_rows, shift, rel_check, info);
+    chol_reg_body(lds, n, G, ldg, Dinv, m_rows, shift, rel_check, info, scales);
+}
+
+// {tol, shift} of the order-n G from its whole diagonal, for the two block
+// factorisations of an order in (272, 544].
+__global__ __launch_bounds__(NT) void chol_scales_kernel(int n, const double* __restrict__ G,
+                                                         int ldg, int m_rows, int shift,
+                                                         int rel_check,
+                                                         double* __restrict__ scales) {
+    double tol, sh;
+    chol_scales(n, G, ldg, m_rows, shift, rel_check, tol, sh);
+    if (threadIdx.x == 0) {
+        scales[0] = tol;
+        scales[1] = sh;
+    }
+}
+
+// Status of the two block factorisations as one order-n status: the first
+// failing pivot of the leading block, else n1 + that of the trailing one.
+__global__ void chol_info_combine_kernel(const int* __restrict__ parts, int n1,
+                                         int* __restrict__ info) {
+    if (threadIdx.x == 0) *info = parts[0] ? parts[0] : (parts[1] ? n1 + parts[1] : 0);
 }
 
 __global__ __launch_bounds__(NT) void chol_reg_batched_kernel(const DhSpdJob* __restrict__ jobs,

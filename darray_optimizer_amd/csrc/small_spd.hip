@@ -39,12 +39,37 @@ int dh_gram_reg_batched(dh_handle_t h, const DhGramRegJob* jobs, int count,
 int dh_chol_small_launch(dh_handle_t h, int n, double* G, int ldg, double* Dinv, int m_rows,
                          bool shift, bool rel_check, int* info_dev) {
     if (n > kMaxN) return dh_fail(DH_ERR_UNSUPPORTED, "chol_small: order %d > %d", n, kMaxN);
-    if (n <= 16 * kRegBlocks)
+    if (n <= 16 * kRegBlocks) {
         chol_reg_kernel<<<1, NT, 0, h->stream>>>(n, G, ldg, Dinv, m_rows, shift ? 1 : 0,
-                                                 rel_check ? 1 : 0, info_dev);
-    else
-        chol_small_kernel<<<1, NT, 0, h->stream>>>(n, G, ldg, Dinv, m_rows, shift ? 1 : 0,
-                                                   rel_check ? 1 : 0, info_dev);
+                                                 rel_check ? 1 : 0, info_dev, nullptr);
+        DH_CHECK_LAUNCH();
+        return 0;
+    }
+    // 272 < n <= 544 as two register Cholesky factorisations: with G = [A11 A12;
+    // . A22] and A11 of order 272, U11 from A11, U12 = U11^-T A12, then U22 from
+    // the Schur complement A22 - U12^T U12.  Both use the breakdown tolerance
+    // and shift of the whole G; a breakdown in the first leaves the rest to
+    // run on garbage, and the combined status reports it (the caller redoes
+    // the factorisation then).
+    const int n1 = 16 * kRegBlocks, n2 = n - n1;
+    double* sc = (double*)dh_ws(h, DH_WS_CHOL2, 64);
+    if (!sc) return DH_ERR_ALLOC;
+    int* parts = (int*)(sc + 2);
+    chol_scales_kernel<<<1, NT, 0, h->stream>>>(n, G, ldg, m_rows, shift ? 1 : 0,
+                                                rel_check ? 1 : 0, sc);
+    DH_CHECK_LAUNCH();
+    chol_reg_kernel<<<1, NT, 0, h->stream>>>(n1, G, ldg, Dinv, m_rows, shift ? 1 : 0,
+                                             rel_check ? 1 : 0, parts, sc);
+    DH_CHECK_LAUNCH();
+    double* A12 = G + (size_t)n1 * ldg;
+    double* A22 = A12 + n1;
+    DH_CHECK_STATUS(dh_trsm_impl<double>(h, 'L', 'T', false, n1, n2, 1.0, G, ldg, A12, ldg));
+    DH_CHECK_STATUS(dh_gemm_impl<double>(h, true, false, n2, n2, n1, -1.0, A12, ldg, A12, ldg,
+                                         1.0, A22, ldg, 1));
+    chol_reg_kernel<<<1, NT, 0, h->stream>>>(n2, A22, ldg, Dinv + 256 * kRegBlocks, m_rows,
+                                             shift ? 1 : 0, rel_check ? 1 : 0, parts + 1, sc);
+    DH_CHECK_LAUNCH();
+    chol_info_combine_kernel<<<1, 64, 0, h->stream>>>(parts, n1, info_dev);
     DH_CHECK_LAUNCH();
     return 0;
 }

@@ -331,19 +331,21 @@ def test_cholqr2_breakdown(dev, n):
     assert dev.cholqr2(dev_mat(Y)) > 0
 
 
-def test_cholqr_shifted_fallback_ill_conditioned(dev):
+@pytest.mark.parametrize("n", [30, 400])
+def test_cholqr_shifted_fallback_ill_conditioned(dev, n):
     # kappa(Y) = 1e11 is beyond CholeskyQR2 (~u^{-1/2}); the shifted
-    # CholeskyQR3 fallback must still return an orthonormal Q with QR = Y.
+    # CholeskyQR3 fallback must still return an orthonormal Q with QR = Y
+    # (n = 400: the shifted Cholesky as two order-272 register blocks).
     import torch
     rng = np.random.default_rng(18)
-    U, _ = np.linalg.qr(rng.standard_normal((4000, 30)))
-    V, _ = np.linalg.qr(rng.standard_normal((30, 30)))
-    Y = U @ np.diag(np.logspace(0, -11, 30)) @ V.T
+    U, _ = np.linalg.qr(rng.standard_normal((4000, n)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    Y = U @ np.diag(np.logspace(0, -11, n)) @ V.T
     Yd = dev_mat(Y)
-    R = empty_mat(30, 30, torch.float64)
+    R = empty_mat(n, n, torch.float64)
     assert dev.cholqr2(Yd, R) == 0
     Q, Rh = dh.from_dev(Yd), dh.from_dev(R)
-    assert np.linalg.norm(Q.T @ Q - np.eye(30)) < 1e-12
+    assert np.linalg.norm(Q.T @ Q - np.eye(n)) < 1e-12
     assert rel(Q @ Rh, Y) < 1e-13
 
 
