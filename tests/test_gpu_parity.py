@@ -251,14 +251,17 @@ def test_potrf_test3_known_answer(dev):
     np.testing.assert_allclose(np.triu(dh.from_dev(Ad)), m.T, rtol=1e-12, atol=1e-12)
 
 
-@pytest.mark.parametrize("bad", [0, 23, 299])
-def test_potrf_reports_first_bad_minor(dev, bad):
+@pytest.mark.parametrize("bs", [64, 256])
+@pytest.mark.parametrize("bad", [0, 23, 200, 299])
+def test_potrf_reports_first_bad_minor(dev, bad, bs):
+    # bs = 256: the FP64 diagonal block of order 256 is the register
+    # factorisation; the last 44 columns and every bs = 64 block are not.
     rng = np.random.default_rng(15)
     M = rng.standard_normal((300, 300))
     A = M.T @ M + 300 * np.eye(300)
     A[bad, bad] = -1.0
     Ad = dev_mat(A)
-    assert dev.potrf(Ad, 64) == bad + 1
+    assert dev.potrf(Ad, bs) == bad + 1
 
 
 def test_potrf_lookahead_orders_with_the_handle_stream(dev):
