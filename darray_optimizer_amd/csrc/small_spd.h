@@ -382,9 +382,9 @@ __device__ __forceinline__ void trsm_rows_body(int blk, int m, int n, double* __
 // instead of an LDS panel, so LDS does not cap the waves per SIMD.  For each
 // block column J the workgroup stages U[0:16J, 16J:16J+16] in LDS once
 // (coalesced column reads; every wave then reads its B operands there
-// instead of 16 scattered columns from L2), and a 16 x 17 LDS tile per wave
-// turns each solved block from the accumulator layout (row g + 4r, column
-// li) into the operand layout.  Both block loops are unrolled (register
+// instead of 16 scattered columns from L2).  The products are formed
+// transposed, so each solved block comes out of the MFMA already in the
+// operand layout.  Both block loops are unrolled (register
 // arrays need static indices); blocks past n are skipped by a uniform test.
 // Every wave of the workgroup must call this (workgroup barriers); rows
 // past m are neither read nor written.
@@ -392,7 +392,7 @@ template <int NBM, int WAVES>
 __device__ __forceinline__ void trsm_rows_reg_body(int r0, int m, int n, double* __restrict__ Y,
                                                    int ldy, const double* __restrict__ U,
                                                    int ldu, const double* __restrict__ Dinv,
-                                                   double (*tile)[17], double (*sU)[17]) {
+                                                   double (*sU)[17]) {
     constexpr int NT_ = WAVES * 64;
     static_assert(NT_ == 256 || NT_ == 512, "block column J of U: 256 J values over the threads");
     // Values of U's block column J staged per thread (J at 256 threads).
@@ -448,33 +448,27 @@ __device__ __forceinline__ void trsm_rows_reg_body(int r0, int m, int n, double*
                     }
                 }
             }
-            // acc = sum_{K<J} Q_K U_KJ, one accumulator chain per k-slot s.
+            // The products are formed transposed, acc^T = sum_{K<J} U_KJ^T Q_K^T
+            // (one accumulator chain per k-slot s): the accumulator layout of a
+            // transposed product (lane (g, li), value r: row li, column g + 4r of
+            // the untransposed one) is the A-operand layout, so no LDS round trip.
             dh_d4 acc[4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
 #pragma unroll
             for (int K = 0; K < J; ++K)
 #pragma unroll
                 for (int s4 = 0; s4 < 4; ++s4)
-                    acc[s4] = Mfma<double>::op(qa[K][s4], sU[16 * K + 4 * s4 + g][li], acc[s4]);
+                    acc[s4] = Mfma<double>::op(sU[16 * K + 4 * s4 + g][li], qa[K][s4], acc[s4]);
             const dh_d4 a = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-            // acc to the A-operand layout, T = Y_J - acc, Q_J = T U_JJ^{-1}.
-#pragma unroll
-            for (int r = 0; r < 4; ++r) tile[g + 4 * r][li] = a[r];
-            wave_lds_sync();
+            // Q_J^T = U_JJ^{-T} (Y_J - acc)^T, again in the A-operand layout.
             dh_d4 q = {0, 0, 0, 0};
 #pragma unroll
-            for (int ks = 0; ks < 4; ++ks)
-                q = Mfma<double>::op(yc[ks] - tile[li][4 * ks + g], dc[ks], q);
-            wave_lds_sync();
-#pragma unroll
-            for (int r = 0; r < 4; ++r) tile[g + 4 * r][li] = q[r];
-            wave_lds_sync();
+            for (int ks = 0; ks < 4; ++ks) q = Mfma<double>::op(dc[ks], yc[ks] - a[ks], q);
 #pragma unroll
             for (int s4 = 0; s4 < 4; ++s4) {
-                qa[J][s4] = tile[li][4 * s4 + g];
+                qa[J][s4] = q[s4];
                 const int col = 16 * J + 4 * s4 + g;
                 if (rowA < m && col < n) Y[rowA + (size_t)col * ldy] = qa[J][s4];
             }
-            wave_lds_sync();
         }
     }
 }
@@ -490,27 +484,24 @@ __global__ __launch_bounds__(WAVES * 64) void trsm_rows_reg_kernel(int m, int n,
                                                                    const double* __restrict__ Dinv,
                                                                    const int* __restrict__ guard,
                                                                    int nguard) {
-    __shared__ double tile[WAVES][16][17];
     __shared__ double sU[16 * (kRegBlocks - 1)][17];
     // Whole-workgroup exit when an earlier Cholesky broke down.
     if (guard && (guard[0] | (nguard > 1 ? guard[1] : 0))) return;
     const int wave = threadIdx.x >> 6;
     const int r0 = (blockIdx.x * WAVES + wave) * 16;
-    trsm_rows_reg_body<kRegBlocks, WAVES>(r0, m, n, Y, ldy, U, ldu, Dinv, tile[wave], sU);
+    trsm_rows_reg_body<kRegBlocks, WAVES>(r0, m, n, Y, ldy, U, ldu, Dinv, sU);
 }
 
 template <int WAVES>
 __global__ __launch_bounds__(WAVES * 64) void trsm_rows_reg_batched_kernel(
     const DhSpdJob* __restrict__ jobs) {
-    __shared__ double tile[WAVES][16][17];
     __shared__ double sU[16 * (kRegBlocks - 1)][17];
     const DhSpdJob& j = jobs[blockIdx.y];
     // Whole-workgroup exit: the body synchronises the workgroup.
     if (blockIdx.x * WAVES * 16 >= (unsigned)j.m || *j.info != 0) return;
     const int wave = threadIdx.x >> 6;
     const int r0 = (blockIdx.x * WAVES + wave) * 16;
-    trsm_rows_reg_body<kRegBlocks, WAVES>(r0, j.m, j.n, j.Y, j.ldy, j.G, j.ldg, j.Dinv,
-                                          tile[wave], sU);
+    trsm_rows_reg_body<kRegBlocks, WAVES>(r0, j.m, j.n, j.Y, j.ldy, j.G, j.ldg, j.Dinv, sU);
 }
 
 template <int WAVES>
