@@ -883,11 +883,11 @@ __global__ __launch_bounds__(NT) void chol_reg_batched_kernel(const DhSpdJob* __
 // CholeskyQR product, whose 64 x 64-tiled GEMM would pad 272 to 320.  A
 // workgroup (8 waves) takes a slab of rows and accumulates all upper 16 x 16
 // blocks of the order-16*kRegBlocks Gram in registers (FP64 MFMA accumulator
-// layout); block b (numbered row by row, 153 of them) belongs to wave b % 8,
+// layout); block b (153 of them, dealt by kGramMap) belongs to wave b % 8,
 // fixed at compile time so every operand index is static.  The slab streams
 // through LDS in 16-row chunks (double buffered, the next chunk's loads in
-// flight during the MFMAs); per 4-row step a wave reads the 17 column
-// operands once and issues its blocks' MFMAs (A = B layout: lane (g, li)
+// flight during the MFMAs); per 4-row step a wave reads the column operands
+// its blocks use once and issues their MFMAs (A = B layout: lane (g, li)
 // holds Y(k + g, 16 c + li) for column block c).  Partial sums go to
 // part[z][block][r][lane] and gram_reduce_kernel adds the slabs.
 constexpr int kGramBlocks = kRegBlocks * (kRegBlocks + 1) / 2;           // 153
@@ -897,17 +897,45 @@ constexpr int kGramPitch = 16 * kRegBlocks + 4;
 struct GramMap {
     int bi[kGramBlocks], bj[kGramBlocks];
 };
+// Block b = 8 S + W is slot S of wave W.  The blocks are dealt in compact
+// groups (found by a local search over partitions of the 153 blocks): each
+// wave's 19 or 20 blocks use 8 of the 17 column operands, so a wave reads 8
+// operands per 4-row step instead of 17 when dealt row by row.
 __host__ __device__ constexpr GramMap make_gram_map() {
+    constexpr int bi[kGramBlocks] = {
+        0, 2, 1, 1, 1, 0, 0, 0, 0, 2, 1, 1, 1, 0, 0, 0, 0, 2, 1, 1, 1, 0, 0, 2, 0, 2, 1, 1, 1, 0,
+        4, 2, 0, 2, 3, 1, 2, 0, 4, 2, 0, 4, 3, 3, 2, 1, 4, 2, 0, 4, 3, 3, 5, 1, 4, 3, 5, 4, 3, 3,
+        5, 1, 4, 3, 5, 4, 7, 3, 5, 2, 6, 3, 5, 4, 7, 4, 5, 2, 6, 7, 5, 5, 7, 4, 8, 2, 6, 7, 6, 5,
+        7, 4, 8, 2, 8, 7, 6, 6, 10, 6, 8, 3, 8, 7, 6, 6, 10, 6, 10, 3, 9, 8, 8, 6, 10, 9, 10, 3,
+        9, 8, 8, 7, 12, 9, 10, 5, 9, 11, 10, 7, 12, 9, 11, 5, 9, 11, 12, 9, 13, 13, 11, 12, 11,
+        15, 14, 14, 13, 14, 13, 12, 11, 16, 15};
+    constexpr int bj[kGramBlocks] = {
+        0, 4, 3, 4, 1, 1, 4, 7, 5, 6, 7, 6, 8, 2, 6, 11, 8, 9, 15, 9, 11, 3, 9, 3, 10, 14, 16, 10,
+        14, 13, 6, 7, 12, 16, 3, 13, 8, 16, 8, 11, 14, 5, 7, 4, 10, 2, 11, 15, 15, 7, 10, 6, 5, 5,
+        12, 8, 6, 9, 12, 9, 10, 12, 15, 11, 8, 14, 10, 14, 11, 2, 8, 15, 12, 16, 12, 4, 14, 5, 9,
+        7, 15, 7, 13, 10, 8, 12, 11, 8, 6, 9, 16, 13, 10, 13, 9, 11, 12, 7, 12, 10, 13, 5, 11, 15,
+        15, 14, 15, 13, 11, 13, 9, 15, 12, 16, 16, 10, 13, 16, 11, 16, 14, 9, 13, 13, 14, 13, 12,
+        11, 10, 14, 15, 14, 13, 16, 15, 16, 14, 16, 15, 14, 14, 12, 12, 16, 15, 16, 16, 14, 13,
+        16, 15, 16, 15};
     GramMap m{};
-    int b = 0;
-    for (int i = 0; i < kRegBlocks; ++i)
-        for (int j = i; j < kRegBlocks; ++j) {
-            m.bi[b] = i;
-            m.bj[b] = j;
-            ++b;
-        }
+    for (int b = 0; b < kGramBlocks; ++b) {
+        m.bi[b] = bi[b];
+        m.bj[b] = bj[b];
+    }
     return m;
 }
+// Every upper block (bi <= bj < kRegBlocks) appears exactly once.
+__host__ __device__ constexpr bool gram_map_is_partition() {
+    constexpr GramMap m = make_gram_map();
+    for (int i = 0; i < kRegBlocks; ++i)
+        for (int j = i; j < kRegBlocks; ++j) {
+            int hits = 0;
+            for (int b = 0; b < kGramBlocks; ++b) hits += (m.bi[b] == i && m.bj[b] == j);
+            if (hits != 1) return false;
+        }
+    return true;
+}
+static_assert(gram_map_is_partition(), "Gram block map must cover the upper triangle once");
 constexpr GramMap kGramMap = make_gram_map();
 
 // acc[S] += Y_bi^T Y_bj for the blocks of wave W (indices fixed at compile time).
