@@ -51,6 +51,7 @@ struct dh_handle_s {
     // the blocked Cholesky; created on first use, released by dh_destroy.
     hipStream_t aux = nullptr;
     hipEvent_t ev_panel = nullptr, ev_update = nullptr;
+    hipEvent_t ev_orth = nullptr;  // end of a queued range-finder orthogonalisation
     // Pinned host words for device status reads (created on first use).
     int* host_status = nullptr;
     // Resident workgroups per CU of the persistent kernels, by kernel

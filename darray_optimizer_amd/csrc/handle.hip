@@ -96,6 +96,7 @@ extern "C" int dh_destroy(dh_handle_t h) {
         if (h->ws[s]) (void)hipFree(h->ws[s]);
     if (h->ev_panel) (void)hipEventDestroy(h->ev_panel);
     if (h->ev_update) (void)hipEventDestroy(h->ev_update);
+    if (h->ev_orth) (void)hipEventDestroy(h->ev_orth);
     if (h->aux) (void)hipStreamDestroy(h->aux);
     if (h->host_status) (void)hipHostFree(h->host_status);
     delete static_cast<std::map<std::pair<const void*, int>, int>*>(h->occupancy);

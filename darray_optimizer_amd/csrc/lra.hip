@@ -277,18 +277,74 @@ __global__ __launch_bounds__(256) void random_column_kernel(int m, double* __res
 // the column at the failing pivot is replaced by a pseudo-random vector and
 // the orthogonalisation repeated: the basis spans range(Y) plus random
 // directions, which can only enlarge the subspace the range finder keeps.
-int dh_range_orth(dh_handle_t h, int m, int n, double* Y, int ldy) {
-    for (int attempt = 0; attempt <= n; ++attempt) {
+static int replace_column(dh_handle_t h, int m, double* Y, int ldy, int attempt, int piv) {
+    uint64_t seed = 0xD1B54A32D192ED03ull * (uint64_t)(attempt + 1) + (uint64_t)piv;
+    random_column_kernel<<<max(1, min(dh_cdiv(m, 256), 1024)), 256, 0, h->stream>>>(
+        m, Y + (size_t)(piv - 1) * ldy, seed);
+    DH_CHECK_LAUNCH();
+    return 0;
+}
+
+static int range_orth_from(dh_handle_t h, int m, int n, double* Y, int ldy, int first) {
+    for (int attempt = first; attempt <= n; ++attempt) {
         int st = dh_cholqr2_impl(h, m, n, Y, ldy, nullptr, 0);
         if (st <= 0) return st;
         int piv = st % 1000000;
         if (piv < 1 || piv > n) return st;
-        uint64_t seed = 0xD1B54A32D192ED03ull * (uint64_t)(attempt + 1) + (uint64_t)piv;
-        random_column_kernel<<<max(1, min(dh_cdiv(m, 256), 1024)), 256, 0, h->stream>>>(
-            m, Y + (size_t)(piv - 1) * ldy, seed);
-        DH_CHECK_LAUNCH();
+        DH_CHECK_STATUS(replace_column(h, m, Y, ldy, attempt, piv));
     }
     return dh_fail(n, "range finder: basis of %d columns could not be completed", n);
+}
+
+int dh_range_orth(dh_handle_t h, int m, int n, double* Y, int ldy) {
+    return range_orth_from(h, m, n, Y, ldy, 0);
+}
+
+// dh_range_orth with the status check deferred, so that the caller can queue
+// the next K-pass before the host waits: both CholeskyQR sweeps and the copy of
+// their statuses to pinned host memory are queued, then an event.  Orders
+// above dh_chol_small_max_order() (or no pinned memory) take the synchronous
+// path and leave nothing pending.
+int dh_range_orth_begin(dh_handle_t h, int m, int n, double* Y, int ldy, int* pending) {
+    *pending = 0;
+    int* pinned = dh_host_status(h);
+    if (n == 0 || n > dh_chol_small_max_order() || !pinned) return dh_range_orth(h, m, n, Y, ldy);
+    if (!h->ev_orth) DH_CHECK_HIP(hipEventCreateWithFlags(&h->ev_orth, hipEventDisableTiming));
+    const size_t nn = (size_t)n * n;
+    double* G = (double*)dh_ws(h, DH_WS_GRAM, sizeof(double) * nn * 5);
+    if (!G) return DH_ERR_ALLOC;
+    double* Dbuf = (double*)dh_ws(h, DH_WS_RBUF, sizeof(double) * 256 * (size_t)dh_cdiv(n, 16));
+    if (!Dbuf) return DH_ERR_ALLOC;
+    int* dinfo = (int*)dh_ws(h, DH_WS_INFO, 256);
+    if (!dinfo) return DH_ERR_ALLOC;
+    DH_CHECK_STATUS(cholqr_sweep_small_async(h, m, n, Y, ldy, G, Dbuf, dinfo, 0));
+    DH_CHECK_STATUS(cholqr_sweep_small_async(h, m, n, Y, ldy, G, Dbuf, dinfo, 1));
+    DH_CHECK_HIP(hipMemcpyAsync(pinned + 2, dinfo, 2 * sizeof(int), hipMemcpyDeviceToHost,
+                                h->stream));
+    DH_CHECK_HIP(hipEventRecord(h->ev_orth, h->stream));
+    *pending = 1;
+    return 0;
+}
+
+// Completes dh_range_orth_begin: waits for the statuses only.  After a
+// breakdown it waits for the stream and finishes on the synchronous path as
+// dh_range_orth would have (a failed first sweep left Y untouched; a failed
+// second sweep left the first sweep's Q, as after attempt 0), and sets
+// *redone: work queued after the begin read a Y that has changed since.
+int dh_range_orth_finish(dh_handle_t h, int m, int n, double* Y, int ldy, int* pending,
+                         int* redone) {
+    *redone = 0;
+    if (!*pending) return 0;
+    *pending = 0;
+    DH_CHECK_HIP(hipEventSynchronize(h->ev_orth));
+    const int* hinfo = dh_host_status(h) + 2;
+    const int s0 = hinfo[0], s1 = hinfo[1];
+    if (s0 == 0 && s1 == 0) return 0;
+    DH_CHECK_HIP(hipStreamSynchronize(h->stream));
+    *redone = 1;
+    if (s0 != 0) return range_orth_from(h, m, n, Y, ldy, 0);
+    DH_CHECK_STATUS(replace_column(h, m, Y, ldy, 0, s1));
+    return range_orth_from(h, m, n, Y, ldy, 1);
 }
 
 extern "C" int dh_cholqr2_f64(dh_handle_t h, int m, int n, double* Y, int ldy, double* R,

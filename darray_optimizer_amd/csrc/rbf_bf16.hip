@@ -541,8 +541,13 @@ extern "C" int dh_rbf_project_f16(dh_handle_t h, int m, int n, int d, int l, con
 }
 
 int dh_range_orth(dh_handle_t h, int m, int n, double* Y, int ldy);
+int dh_range_orth_begin(dh_handle_t h, int m, int n, double* Y, int ldy, int* pending);
+int dh_range_orth_finish(dh_handle_t h, int m, int n, double* Y, int ldy, int* pending,
+                         int* redone);
 
 // Mixed-precision passes of dh_lra: 16-bit (BF16 or FP16) K-passes, FP64 CholeskyQR2.
+// Each orthogonalisation's status is checked only after the next pass is queued
+// (no idle GPU while the host waits); a breakdown, rare, redoes that pass.
 template <bool F16>
 int lra_16_passes(dh_handle_t h, int n, int d, int l, int q, const double* X, int ldx,
                   const double* y, double gamma, const double* Omega, int ldo, double* Q, int ldq,
@@ -556,23 +561,34 @@ int lra_16_passes(dh_handle_t h, int n, int d, int l, int q, const double* X, in
     const float c2 = (float)(2.0 * g) * kLog2e;
     DH_CHECK_STATUS(pack_omega<F16>(h, n, l, Omega, ldo, y, Op));
     DH_CHECK_STATUS(project_16<F16>(h, n, n, d, l, p, p, c2, Op, y, Q, ldq));
-    DH_CHECK_STATUS(dh_range_orth(h, n, l, Q, ldq));
+    int pending = 0, redone = 0;
+    DH_CHECK_STATUS(dh_range_orth_begin(h, n, l, Q, ldq, &pending));
     // Power iterations alternate the basis between Q and W (no copy for even q).
     double *cur = Q, *nxt = W;
     int ldc = ldq, ldn = ldw;
     for (int it = 0; it < q; ++it) {
-        DH_CHECK_STATUS(pack_omega<F16>(h, n, l, cur, ldc, y, Op));
-        DH_CHECK_STATUS(project_16<F16>(h, n, n, d, l, p, p, c2, Op, y, nxt, ldn));
-        DH_CHECK_STATUS(dh_range_orth(h, n, l, nxt, ldn));
+        do {
+            DH_CHECK_STATUS(pack_omega<F16>(h, n, l, cur, ldc, y, Op));
+            DH_CHECK_STATUS(project_16<F16>(h, n, n, d, l, p, p, c2, Op, y, nxt, ldn));
+            DH_CHECK_STATUS(dh_range_orth_finish(h, n, l, cur, ldc, &pending, &redone));
+        } while (redone);
+        DH_CHECK_STATUS(dh_range_orth_begin(h, n, l, nxt, ldn, &pending));
         std::swap(cur, nxt);
         std::swap(ldc, ldn);
     }
+    // The copy is queued after the pending orthogonalisation, so Q holds its
+    // result (or, after a breakdown, the state the fallback starts from) even
+    // though the last pass then overwrites W.
     if (cur != Q)
         DH_CHECK_HIP(hipMemcpy2DAsync(Q, sizeof(double) * (size_t)ldq, cur,
                                       sizeof(double) * (size_t)ldc, sizeof(double) * (size_t)n, l,
                                       hipMemcpyDeviceToDevice, h->stream));
-    DH_CHECK_STATUS(pack_omega<F16>(h, n, l, Q, ldq, y, Op));
-    return project_16<F16>(h, n, n, d, l, p, p, c2, Op, y, W, ldw);
+    do {
+        DH_CHECK_STATUS(pack_omega<F16>(h, n, l, Q, ldq, y, Op));
+        DH_CHECK_STATUS(project_16<F16>(h, n, n, d, l, p, p, c2, Op, y, W, ldw));
+        DH_CHECK_STATUS(dh_range_orth_finish(h, n, l, Q, ldq, &pending, &redone));
+    } while (redone);
+    return 0;
 }
 
 int dh_lra_bf16_passes(dh_handle_t h, int n, int d, int l, int q, const double* X, int ldx,

@@ -159,6 +159,34 @@ def test_lra_mixed_error_close_to_fp64(dev, labels, q):
     assert errs[dh.DH_PREC_F16] <= 1.2 * errs[dh.DH_PREC_F64] + 5e-4
 
 
+@pytest.mark.parametrize("q", [0, 1, 2])
+@pytest.mark.parametrize("prec", ["BF16", "F16"])
+def test_lra_mixed_breakdown_redoes_the_queued_pass(dev, prec, q):
+    """Two equal columns of Omega make the first Y exactly rank deficient in
+    any precision, so the first orthogonalisation breaks down.  The 16-bit
+    range finder checks that status only after queuing the next pass: the
+    fallback then completes the basis and the pass is redone.  q = 0 covers
+    the final pass, q = 1 the copy of W into Q, q = 2 a power iteration."""
+    import torch
+    n, d, l = 3000, 16, 24
+    X, _ = dh.fill_mixture(12, d, n, 4, 0.3, labels=True)
+    Om = np.array(dh.fill_normal(13, n, l))
+    Om[:, 5] = Om[:, 2]
+    Xd, Omd = dev_pts(X), dh.to_dev(Om)
+    errs = {}
+    for p in (dh.DH_PREC_F64, getattr(dh, "DH_PREC_" + prec)):
+        Q = torch.empty((l, n), dtype=torch.float64, device="cuda")
+        W = torch.empty((l, n), dtype=torch.float64, device="cuda")
+        assert dev.lra(Xd, Omd, 1.0 / d, q, Q, W, prec=p) == 0
+        QtQ = (Q @ Q.T).cpu().numpy()
+        assert np.linalg.norm(QtQ - np.eye(l)) < 1e-12
+        e2, k2 = dev.lra_error(Xd, 1.0 / d, Q, W)
+        errs[p] = np.sqrt(e2 / k2)
+    e64, emix = errs[dh.DH_PREC_F64], errs[getattr(dh, "DH_PREC_" + prec)]
+    print(f"{prec} q={q}: error {emix:.3e} (FP64 {e64:.3e})", flush=True)
+    assert emix <= 1.5 * e64 + 3e-3
+
+
 def test_kernel_timing_counts_projection_launches(dev):
     import torch
     n, d, l = 1024, 64, 40
